@@ -1,0 +1,246 @@
+"""dumato_amd: MI355X graph pattern mining (k-clique counts, k-motif histograms).
+
+Thin ctypes binding of libdumato_amd.so (C ABI in include/dumato.h).  All
+counting runs in the library's HIP kernels; this module only marshals
+arguments.  There is no CPU fallback: if the library or a GPU is missing, the
+calls raise.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libdumato_amd.so")
+
+DM_SCHED_PLAIN = 0
+DM_SCHED_LEARNED = 1
+
+_ERR = {-1: "DM_ERR_ARG", -2: "DM_ERR_IO", -3: "DM_ERR_HIP", -4: "DM_ERR_UNSUPPORTED",
+        -5: "DM_ERR_NOMEM", -6: "DM_ERR_DEVICE"}
+
+
+class DMError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("%s: %s" % (_ERR.get(code, code), msg))
+        self.code = code
+
+
+class Config(ctypes.Structure):
+    """Mirror of struct dm_config."""
+    _fields_ = [("sched", ctypes.c_int32), ("vwave", ctypes.c_int32), ("steal", ctypes.c_int32),
+                ("steal_min", ctypes.c_int32), ("steal_period", ctypes.c_int32),
+                ("lds_bytes", ctypes.c_int32), ("waves_per_block", ctypes.c_int32),
+                ("blocks", ctypes.c_int32), ("record_costs", ctypes.c_int32),
+                ("per_root", ctypes.c_int32), ("root_begin", ctypes.c_int32),
+                ("root_end", ctypes.c_int32), ("watchdog_s", ctypes.c_double)]
+
+
+class Stats(ctypes.Structure):
+    """Mirror of struct dm_stats."""
+    _fields_ = [("device_ms", ctypes.c_double), ("wall_ms", ctypes.c_double),
+                ("sched_ms", ctypes.c_double), ("subgraphs", ctypes.c_uint64),
+                ("alg_bytes", ctypes.c_uint64), ("donations", ctypes.c_uint64),
+                ("jobs", ctypes.c_uint64), ("vwave", ctypes.c_int32), ("workers", ctypes.c_int32),
+                ("blocks", ctypes.c_int32), ("lds_bytes", ctypes.c_int32)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+_lib = None
+
+i32p = ctypes.POINTER(ctypes.c_int32)
+i64p = ctypes.POINTER(ctypes.c_int64)
+u64p = ctypes.POINTER(ctypes.c_uint64)
+u16p = ctypes.POINTER(ctypes.c_uint16)
+f32p = ctypes.POINTER(ctypes.c_float)
+GP = ctypes.c_void_p
+
+# (name, restype, argtypes) of every exported entry point (include/dumato.h)
+SYMBOLS = [
+    ("dm_version", ctypes.c_int, []),
+    ("dm_last_error", ctypes.c_char_p, []),
+    ("dm_config_init", ctypes.c_int, [ctypes.POINTER(Config)]),
+    ("dm_graph_load_edgelist", ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(GP)]),
+    ("dm_graph_from_edges", ctypes.c_int, [i32p, i32p, ctypes.c_size_t, ctypes.c_int32, ctypes.POINTER(GP)]),
+    ("dm_graph_info", ctypes.c_int, [GP, i32p, i64p, i32p]),
+    ("dm_graph_csr", ctypes.c_int, [GP, i64p, ctypes.c_size_t, i32p, ctypes.c_size_t]),
+    ("dm_graph_free", None, [GP]),
+    ("dm_gen_erdos_renyi", ctypes.c_int, [ctypes.c_int32, ctypes.c_int64, ctypes.c_uint64, ctypes.POINTER(GP)]),
+    ("dm_gen_chung_lu", ctypes.c_int, [ctypes.c_int32, ctypes.c_int64, ctypes.c_double, ctypes.c_double,
+                                       ctypes.c_uint64, ctypes.POINTER(GP)]),
+    ("dm_gen_rmat", ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.c_double, ctypes.c_double,
+                                   ctypes.c_double, ctypes.c_uint64, ctypes.POINTER(GP)]),
+    ("dm_gen_barabasi_albert", ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                              ctypes.c_int32, ctypes.c_uint64, ctypes.POINTER(GP)]),
+    ("dm_clique_count", ctypes.c_int, [GP, ctypes.c_int, ctypes.POINTER(Config), u64p]),
+    ("dm_motif_count", ctypes.c_int, [GP, ctypes.c_int, ctypes.POINTER(Config), u64p, ctypes.c_size_t, u64p]),
+    ("dm_last_stats", ctypes.c_int, [GP, ctypes.POINTER(Stats)]),
+    ("dm_last_root_costs", ctypes.c_int, [GP, u64p, ctypes.c_size_t]),
+    ("dm_last_root_counts", ctypes.c_int, [GP, u64p, ctypes.c_size_t]),
+    ("dm_motif_num_classes", ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ctypes.c_size_t)]),
+    ("dm_pattern_table", ctypes.c_int, [ctypes.c_int, u16p, ctypes.c_size_t, u64p, ctypes.c_size_t]),
+    ("dm_sched_train", ctypes.c_int, [GP, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_uint64,
+                                      ctypes.POINTER(ctypes.c_double)]),
+    ("dm_sched_predict", ctypes.c_int, [GP, f32p, ctypes.c_size_t]),
+]
+
+
+def lib():
+    """Load libdumato_amd.so (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise OSError("libdumato_amd.so not built (run __graft_entry__.build())")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, res, args in SYMBOLS:
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise DMError(rc, lib().dm_last_error().decode(errors="replace"))
+
+
+def _ptr(a, t):
+    return a.ctypes.data_as(t)
+
+
+def config(**kw):
+    """A dm_config with the library defaults, overridden by keyword."""
+    c = Config()
+    _check(lib().dm_config_init(ctypes.byref(c)))
+    for k, v in kw.items():
+        if not hasattr(c, k):
+            raise KeyError(k)
+        setattr(c, k, v)
+    return c
+
+
+class Graph:
+    """Owns a dm_graph handle (host CSR + device copy on first use)."""
+
+    def __init__(self, handle):
+        self._h = GP(handle)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value and _lib is not None:
+            _lib.dm_graph_free(h)
+            self._h = None
+
+    @classmethod
+    def _make(cls, fn, *args):
+        h = GP()
+        _check(fn(*args, ctypes.byref(h)))
+        return cls(h.value)
+
+    @classmethod
+    def from_edgelist(cls, path):
+        return cls._make(lib().dm_graph_load_edgelist, str(path).encode())
+
+    @classmethod
+    def from_edges(cls, src, dst, n=0):
+        s = np.ascontiguousarray(src, dtype=np.int32)
+        d = np.ascontiguousarray(dst, dtype=np.int32)
+        return cls._make(lib().dm_graph_from_edges, _ptr(s, i32p), _ptr(d, i32p), len(s), int(n))
+
+    @classmethod
+    def erdos_renyi(cls, n, m, seed):
+        return cls._make(lib().dm_gen_erdos_renyi, n, m, seed)
+
+    @classmethod
+    def chung_lu(cls, n, m, gamma, wmax, seed):
+        return cls._make(lib().dm_gen_chung_lu, n, m, gamma, wmax, seed)
+
+    @classmethod
+    def rmat(cls, scale, edge_factor, a, b, c, seed):
+        return cls._make(lib().dm_gen_rmat, scale, edge_factor, a, b, c, seed)
+
+    @classmethod
+    def barabasi_albert(cls, n, m, planted, kmin, kmax, seed):
+        return cls._make(lib().dm_gen_barabasi_albert, n, m, planted, kmin, kmax, seed)
+
+    def info(self):
+        n = ctypes.c_int32()
+        m = ctypes.c_int64()
+        md = ctypes.c_int32()
+        _check(lib().dm_graph_info(self._h, ctypes.byref(n), ctypes.byref(m), ctypes.byref(md)))
+        return n.value, m.value, md.value
+
+    @property
+    def n(self):
+        return self.info()[0]
+
+    @property
+    def m(self):
+        return self.info()[1]
+
+    def csr(self):
+        n, m, _ = self.info()
+        off = np.zeros(n + 1, np.int64)
+        adj = np.zeros(max(2 * m, 1), np.int32)
+        _check(lib().dm_graph_csr(self._h, _ptr(off, i64p), len(off), _ptr(adj, i32p), len(adj)))
+        return off, adj[:2 * m]
+
+    # ---- counting ----
+    def clique_count(self, k, cfg=None):
+        out = ctypes.c_uint64()
+        _check(lib().dm_clique_count(self._h, int(k), ctypes.byref(cfg) if cfg else None, ctypes.byref(out)))
+        return int(out.value)
+
+    def motif_count(self, k, cfg=None):
+        ncls = num_classes(k)
+        hist = np.zeros(ncls, np.uint64)
+        tot = ctypes.c_uint64()
+        _check(lib().dm_motif_count(self._h, int(k), ctypes.byref(cfg) if cfg else None,
+                                    _ptr(hist, u64p), ncls, ctypes.byref(tot)))
+        return int(tot.value), hist
+
+    def last_stats(self):
+        s = Stats()
+        _check(lib().dm_last_stats(self._h, ctypes.byref(s)))
+        return s.as_dict()
+
+    def last_root_costs(self):
+        out = np.zeros(max(self.n, 1), np.uint64)
+        _check(lib().dm_last_root_costs(self._h, _ptr(out, u64p), len(out)))
+        return out[:self.n]
+
+    def last_root_counts(self):
+        out = np.zeros(max(self.n, 1), np.uint64)
+        _check(lib().dm_last_root_counts(self._h, _ptr(out, u64p), len(out)))
+        return out[:self.n]
+
+    # ---- learned scheduler ----
+    def sched_train(self, motif, k_train, epochs=30, seed=1):
+        loss = ctypes.c_double()
+        _check(lib().dm_sched_train(self._h, 1 if motif else 0, int(k_train), int(epochs), int(seed),
+                                    ctypes.byref(loss)))
+        return loss.value
+
+    def sched_predict(self):
+        out = np.zeros(max(self.n, 1), np.float32)
+        _check(lib().dm_sched_predict(self._h, _ptr(out, f32p), len(out)))
+        return out[:self.n]
+
+
+def num_classes(k):
+    n = ctypes.c_size_t()
+    _check(lib().dm_motif_num_classes(int(k), ctypes.byref(n)))
+    return int(n.value)
+
+
+def pattern_table(k):
+    """(quick -> class id, class id -> canonical code) regenerated by the library."""
+    ncls = num_classes(k)
+    nq = 1 << ((k + 1) * (k - 2) // 2)
+    q2c = np.zeros(nq, np.uint16)
+    codes = np.zeros(ncls, np.uint64)
+    _check(lib().dm_pattern_table(int(k), _ptr(q2c, u16p), nq, _ptr(codes, u64p), ncls))
+    return q2c, codes

@@ -1,0 +1,860 @@
+// DFS-wide subgraph enumeration on gfx950: k-clique counting and k-motif
+// histograms.
+//
+// Reference semantics (what is computed):
+//   clique  src/main/clique_PA.cu:9-32 with extend_clique / forward / backward /
+//           aggregate_counter (src/main/DuMatoGPU.cu:1194-1220, 836-844, 745-750,
+//           1769-1771): C_{l+1} = N+(last) & C_l, the last level adds |C|.
+//   motif   src/main/motifs_DM_WCLB.cu:9-36 with extend(0,k) (:1702-1721),
+//           canonicalFilter2 (:989-1012), forward_induce (:846-869) and
+//           aggregate_pattern (:930-967).
+//
+// How it runs here (MI355X-first, not a translation):
+//   * A worker is a virtual wave of W = 8/16/32/64 lanes of a wave64.  All
+//     cross-lane steps (compaction, counts) use 64-bit ballots masked to the
+//     worker's lanes, mbcnt-style prefix popcounts and shuffles of width W.
+//   * Graph: symmetric CSR in degree-rank ids.  Cliques use the oriented view
+//     N+(v) = the suffix of N(v) above v (ostart[v]), so "u > last" is the
+//     degree orientation.
+//   * Stack levels l = 0..K-3 live in a per-worker LDS slice and spill to a
+//     per-worker global arena when a level does not fit.
+//   * Motif extend keeps, per level, U_l = sorted union of N(id[0..l]) above
+//     id[0], each element with the mask of the lists containing it (bit 7 =
+//     passes the canonical rule).  The valid elements E_l are exactly the
+//     reference's extensions after canonicalFilter2; the mask bits are the
+//     reference's induction bits (forward_induce / aggregate_pattern).
+//   * The last level is never materialised: for each child c of level K-3,
+//     one pass over N(c) (binary search of each element in U_{K-3}) gives, per
+//     mask value, how many leaves are adjacent to c, and the class of every
+//     leaf follows from (mask, adjacency to c) through the class table.
+//   * Work: static per-worker queues of (root, lo, hi) jobs, stealing of
+//     other workers' static jobs, and donation of the lowest level's
+//     remaining children to a device queue when workers are idle (the
+//     reference's donate(), src/main/DuMatoCPU.cu:463-539, done on the device
+//     instead of by the host).
+#include <hip/hip_runtime.h>
+
+#include "internal.h"
+
+namespace dm {
+
+#define DEV __device__ __forceinline__
+
+constexpr int kThreads = 256;
+constexpr int kMaxPm = 32;  // 2^(K-2) mask values for K <= 7
+
+// ---------------------------------------------------------------------------
+// virtual-wave primitives
+template <int W>
+struct Grp {
+    int gl;  // lane within the worker
+    int sh;  // first wave lane of the worker
+    DEV Grp() {
+        int l = __lane_id();
+        gl = l & (W - 1);
+        sh = l & ~(W - 1);
+    }
+    DEV uint64_t ballot(bool p) const {
+        uint64_t b = __ballot(p);
+        if constexpr (W == 64) return b;
+        else return (b >> sh) & ((1ull << W) - 1);
+    }
+    DEV int below(uint64_t m) const { return __popcll(m & ((1ull << gl) - 1)); }
+    DEV int isum(int x) const {
+#pragma unroll
+        for (int o = W / 2; o > 0; o >>= 1) x += __shfl_xor(x, o, W);
+        return x;
+    }
+    DEV unsigned long long lsum(unsigned long long x) const {
+#pragma unroll
+        for (int o = W / 2; o > 0; o >>= 1) x += __shfl_xor(x, o, W);
+        return x;
+    }
+    DEV int bcast(int x, int src) const { return __shfl(x, src, W); }
+    DEV unsigned long long bcast64(unsigned long long x, int src) const {
+        return __shfl(x, src, W);
+    }
+};
+
+template <int W>
+DEV int uni(int x) {
+    if constexpr (W == 64) return __builtin_amdgcn_readfirstlane(x);
+    else return x;
+}
+
+DEV void wsync() { __threadfence_block(); }
+
+DEV int lower_bound(const int32_t *a, int n, int32_t x) {
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (a[mid] < x) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+DEV int quick_off(int s) { return ((s - 2) * (s + 1)) >> 1; }
+
+DEV uint64_t now_ticks() { return wall_clock64(); }
+
+// ---------------------------------------------------------------------------
+// per-worker state in LDS
+struct Lvl {
+    int32_t *uid;     // U_l ids (motif) / C_l (clique)
+    uint8_t *upm;     // U_l masks; nullptr = level 0 (mask 1, valid)
+    int32_t *eid;     // E_l ids (motif)
+    uint8_t *epm;     // E_l masks; nullptr = level 0
+    char *mark_lds;   // allocator tops before this level's buffers
+    char *mark_glb;
+    uint64_t ind;     // induction (quick bits) of the prefix id[0..l]
+    int32_t nu, ne;   // |U_l|, |E_l|
+    int32_t cur, end; // children cursor / end
+    int32_t vtx;      // vertex placed at level l
+    int32_t sdeg;     // sum of degrees of id[0..l] (algorithmic bytes)
+};
+
+// Per-worker LDS record: Lvl lv[nlv], then uint32 n_gt[kMaxPm] and
+// n_int[kMaxPm] (motif leaf-phase counters), then int32 jid[kJobIds] (the
+// prefix of the current job).  nlv = max(K-2, 1).
+struct WorkerLDS {
+    Lvl *lv;
+    uint32_t *n_gt;
+    uint32_t *n_int;
+    int32_t *jid;
+};
+DEV int nlevels(int K) { return K - 2 > 1 ? K - 2 : 1; }
+__host__ __device__ inline size_t worker_lds_bytes(int K) {
+    int nlv = K - 2 > 1 ? K - 2 : 1;
+    return ((size_t)nlv * sizeof(Lvl) + 2 * kMaxPm * sizeof(uint32_t) + kJobIds * 4 + 15) &
+           ~(size_t)15;
+}
+DEV WorkerLDS worker_lds(char *base, int K) {
+    WorkerLDS w;
+    w.lv = (Lvl *)base;
+    w.n_gt = (uint32_t *)(w.lv + nlevels(K));
+    w.n_int = w.n_gt + kMaxPm;
+    w.jid = (int32_t *)(w.n_int + kMaxPm);
+    return w;
+}
+
+template <int W>
+struct Worker {
+    Grp<W> g;
+    const EnumParams &p;
+    WorkerLDS ws;
+    unsigned long long *shist;
+    char *lds_lo, *lds_top, *lds_hi;
+    char *g_lo, *g_top, *g_hi;
+    int K;
+    int32_t v0;
+    int job_lvl;
+    int steps;
+    unsigned long long cnt;    // per-lane partial count of the current job
+    unsigned long long bytes;  // per-lane partial algorithmic bytes
+    unsigned long long donated;
+    uint64_t t_start;
+    bool abort;
+
+    DEV Worker(const EnumParams &p_, WorkerLDS ws_, unsigned long long *sh_, char *lds_slice,
+               int wid)
+        : p(p_), ws(ws_), shist(sh_) {
+        lds_lo = lds_top = lds_slice;
+        lds_hi = lds_slice + p.lds_per_worker;
+        g_lo = g_top = p.arena + (int64_t)wid * p.arena_per_worker;
+        g_hi = g_lo + p.arena_per_worker;
+        K = p.K;
+        steps = 0;
+        cnt = bytes = donated = 0;
+        abort = false;
+        t_start = now_ticks();
+    }
+
+    // ---- helpers -----------------------------------------------------------
+    DEV int deg(int32_t v) const { return (int)(p.off[v + 1] - p.off[v]); }
+    DEV const int32_t *nbr(int32_t v) const { return p.adj + p.off[v]; }
+
+    DEV void fail(int code) {
+        if (g.gl == 0) atomicCAS(&p.sc->error, 0, code);
+        abort = true;
+    }
+
+    DEV char *alloc(int64_t bytes_) {
+        int64_t b = (bytes_ + 15) & ~(int64_t)15;
+        if (lds_top + b <= lds_hi) {
+            char *r = lds_top;
+            lds_top += b;
+            return r;
+        }
+        if (g_top + b <= g_hi) {
+            char *r = g_top;
+            g_top += b;
+            return r;
+        }
+        fail(2);
+        return nullptr;
+    }
+
+    DEV bool watchdog_expired() {
+        return p.watchdog_ticks && (now_ticks() - t_start > p.watchdog_ticks);
+    }
+
+    DEV bool check_abort() {
+        if ((++steps & 255) == 0) {
+            int e = __hip_atomic_load(&p.sc->error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (e || watchdog_expired()) {
+                if (!e) fail(1);
+                abort = true;
+            }
+        }
+        return abort;
+    }
+
+    // ---- donation --------------------------------------------------------------
+    DEV bool push_job(int lvl, int lo, int hi) {
+        int ok = 0;
+        uint32_t slot = 0;
+        if (g.gl == 0) {
+            uint32_t t = __hip_atomic_load(&p.sc->q_tail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (t < p.q_cap && atomicCAS(&p.sc->q_tail, t, t + 1) == t) {
+                ok = 1;
+                slot = t;
+                atomicAdd(&p.sc->outstanding, 1ull);
+                atomicAdd(&p.sc->donations, 1ull);
+            }
+        }
+        ok = g.bcast(ok, 0);
+        if (!ok) return false;
+        slot = (uint32_t)g.bcast((int)slot, 0);
+        int32_t *dst = (int32_t *)&p.q[slot];
+        // lanes 0..15 write the 16 words of the job
+        for (int j = g.gl; j < 16; j += W) {
+            int32_t v;
+            if (j == 0) v = lvl;
+            else if (j == 1) v = lo;
+            else if (j == 2) v = hi;
+            else if (j == 3) v = 0;
+            else v = (j - 4 <= lvl) ? ws.lv[j - 4].vtx : -1;
+            __hip_atomic_store(dst + j, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        __threadfence();
+        if (g.gl == 0)
+            __hip_atomic_store(&p.q_ready[slot], p.epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        return true;
+    }
+
+    // Donate the lowest level's remaining children when workers are idle.
+    // leaf_k / leaf_lo / leaf_i describe the descending leaf-phase loop of
+    // level leaf_k (remaining children [lo, i]); leaf_k < 0 when not in it.
+    DEV void maybe_donate(int top, int leaf_k, int &leaf_lo, int leaf_i) {
+        if (!p.steal || (steps % p.steal_period) != 0) return;
+        int want = 0;
+        if (g.gl == 0) {
+            int idle = __hip_atomic_load(&p.sc->idle, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            uint32_t h = __hip_atomic_load(&p.sc->q_head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            uint32_t t = __hip_atomic_load(&p.sc->q_tail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            int queued = (t > h) ? (int)(t - h) : 0;
+            want = idle > queued;
+        }
+        if (!g.bcast(want, 0)) return;
+        for (int l = job_lvl; l < top; l++) {
+            int cur = ws.lv[l].cur, end = ws.lv[l].end;
+            if (end - cur >= p.steal_min) {
+                int mid = cur + ((end - cur) >> 1);
+                if (push_job(l, mid, end)) {
+                    ws.lv[l].end = mid;
+                    donated++;
+                }
+                return;
+            }
+        }
+        if (leaf_k >= 0 && leaf_i + 1 - leaf_lo >= p.steal_min) {
+            int mid = leaf_lo + ((leaf_i + 1 - leaf_lo) >> 1);
+            if (push_job(leaf_k, leaf_lo, mid)) {
+                leaf_lo = mid;
+                donated++;
+            }
+        }
+    }
+
+    // ---- motif levels ----------------------------------------------------------
+    DEV void motif_root_level(int32_t v) {
+        Lvl &L = ws.lv[0];
+        const int32_t *a = p.adj + p.ostart[v];
+        int n = (int)(p.off[v + 1] - p.ostart[v]);
+        L.uid = const_cast<int32_t *>(a);
+        L.upm = nullptr;
+        L.eid = const_cast<int32_t *>(a);
+        L.epm = nullptr;
+        L.nu = n;
+        L.ne = n;
+        L.vtx = v;
+        L.ind = 0;
+        L.sdeg = deg(v);
+        L.mark_lds = lds_top;
+        L.mark_glb = g_top;
+    }
+
+    DEV uint8_t upm_at(const Lvl &L, int j) const { return L.upm ? L.upm[j] : (uint8_t)0x81; }
+    DEV int epm_at(const Lvl &L, int j) const { return L.epm ? (int)L.epm[j] : 1; }
+
+    // U_{l+1}, E_{l+1} from level l and child c (pmc = c's mask in U_l).
+    DEV void motif_forward(int l, int32_t c, int pmc) {
+        Lvl &P = ws.lv[l];
+        Lvl &N = ws.lv[l + 1];
+        const int l1 = l + 1;
+        const int32_t *nc = nbr(c);
+        const int dc = deg(c);
+        const int s0 = lower_bound(nc, dc, v0 + 1);
+        const int q = dc - s0;
+        const int pnu = P.nu;
+        N.mark_lds = lds_top;
+        N.mark_glb = g_top;
+        const int cap = pnu + q;
+        int32_t *uid = (int32_t *)alloc((int64_t)cap * 4);
+        uint8_t *upm = (uint8_t *)alloc(cap);
+        int32_t *eid = (int32_t *)alloc((int64_t)cap * 4);
+        uint8_t *epm = (uint8_t *)alloc(cap);
+        char *tmark_l = lds_top, *tmark_g = g_top;
+        int32_t *nw = (int32_t *)alloc((int64_t)(q > 0 ? q : 1) * 4);
+        uint8_t *hit = (uint8_t *)alloc(pnu > 0 ? pnu : 1);
+        if (abort) return;
+        for (int j = g.gl; j < pnu; j += W) hit[j] = 0;
+        wsync();
+        // scan N(c) above v0: mark hits in U_l, collect new vertices (sorted)
+        int nnew = 0;
+        for (int base = 0; base < q; base += W) {
+            int t = base + g.gl;
+            bool in = t < q;
+            int32_t x = in ? nc[s0 + t] : 0;
+            bool found = false;
+            if (in) {
+                int pos = lower_bound(P.uid, pnu, x);
+                found = pos < pnu && P.uid[pos] == x;
+                if (found) hit[pos] = 1;
+            }
+            uint64_t m = g.ballot(in && !found);
+            if (in && !found) nw[nnew + g.below(m)] = x;
+            nnew += __popcll(m);
+        }
+        wsync();
+        // merge by ranks: old elements and new ones interleave in id order
+        const uint8_t bit = (uint8_t)(1u << l1);
+        for (int j = g.gl; j < pnu; j += W) {
+            int32_t x = P.uid[j];
+            int o = j + lower_bound(nw, nnew, x);
+            uint8_t pm = upm_at(P, j);
+            if (hit[j]) pm |= bit;
+            bool valid = (pm & 0x80) && x > c;
+            uid[o] = x;
+            upm[o] = (uint8_t)((pm & 0x7f) | (valid ? 0x80 : 0));
+        }
+        for (int t = g.gl; t < nnew; t += W) {
+            int32_t x = nw[t];
+            int o = t + lower_bound(P.uid, pnu, x);
+            uid[o] = x;
+            upm[o] = (uint8_t)(bit | 0x80);
+        }
+        const int nu = pnu + nnew;
+        wsync();
+        int ne = 0;
+        for (int base = 0; base < nu; base += W) {
+            int j = base + g.gl;
+            bool in = j < nu;
+            uint8_t pm = in ? upm[j] : 0;
+            bool v = in && (pm & 0x80);
+            uint64_t m = g.ballot(v);
+            if (v) {
+                int o = ne + g.below(m);
+                eid[o] = uid[j];
+                epm[o] = (uint8_t)(pm & 0x7f);
+            }
+            ne += __popcll(m);
+        }
+        wsync();
+        // release the temporaries
+        lds_top = tmark_l;
+        g_top = tmark_g;
+        N.uid = uid;
+        N.upm = upm;
+        N.eid = eid;
+        N.epm = epm;
+        N.nu = nu;
+        N.ne = ne;
+        N.vtx = c;
+        N.sdeg = P.sdeg + dc;
+        N.ind = (l1 >= 2) ? P.ind + ((uint64_t)pmc << quick_off(l1)) : 0;
+    }
+
+    // children [lo, hi) of level l = K-3, each a node of level K-2 whose
+    // leaves are counted by class without being materialised.
+    DEV void motif_leaf_phase(int l, int lo, int hi) {
+        Lvl &P = ws.lv[l];
+        const int npm = 1 << (l + 1);
+        const int top = 1 << (l + 1);
+        const int s = quick_off(l + 2);
+        const int o1 = quick_off(l + 1);
+        for (int j = g.gl; j < npm; j += W) {
+            ws.n_gt[j] = 0;
+            ws.n_int[j] = 0;
+        }
+        wsync();
+        int gt_local = 0;
+        for (int j = hi + g.gl; j < P.ne; j += W) {
+            atomicAdd(&ws.n_gt[epm_at(P, j)], 1u);
+            gt_local++;
+        }
+        int gt_total = g.isum(gt_local);
+        wsync();
+        const int pnu = P.nu;
+        for (int i = hi - 1; i >= lo; i--) {
+            if (check_abort()) return;
+            maybe_donate(l, l, lo, i);
+            if (i < lo) break;
+            const int32_t c = P.eid[i];
+            const int pmc = epm_at(P, i);
+            const uint64_t indc = P.ind + ((l + 1 >= 2) ? ((uint64_t)pmc << o1) : 0);
+            const int32_t *nc = nbr(c);
+            const int dc = deg(c);
+            const int s0 = lower_bound(nc, dc, v0 + 1);
+            int my_new = 0;
+            for (int t = s0 + g.gl; t < dc; t += W) {
+                int32_t x = nc[t];
+                int pos = lower_bound(P.uid, pnu, x);
+                if (pos < pnu && P.uid[pos] == x) {
+                    uint8_t pm = upm_at(P, pos);
+                    if ((pm & 0x80) && x > c) atomicAdd(&ws.n_int[pm & 0x7f], 1u);
+                } else {
+                    my_new++;
+                }
+            }
+            const int n_new = g.isum(my_new);
+            wsync();
+            for (int j = g.gl; j < npm; j += W) {
+                uint32_t a = ws.n_int[j];
+                uint32_t b = ws.n_gt[j] - a;
+                if (a) atomicAdd(&shist[p.lut[indc + ((uint64_t)(j | top) << s)]], (unsigned long long)a);
+                if (b) atomicAdd(&shist[p.lut[indc + ((uint64_t)j << s)]], (unsigned long long)b);
+                ws.n_int[j] = 0;
+            }
+            if (g.gl == 0) {
+                if (n_new) atomicAdd(&shist[p.lut[indc + ((uint64_t)top << s)]], (unsigned long long)n_new);
+                ws.n_gt[pmc] += 1;
+                const unsigned long long leaves = (unsigned long long)n_new + (unsigned long long)gt_total;
+                cnt += leaves;
+                bytes += 4ull * (unsigned long long)(P.sdeg + dc) + 4ull * (K - 1) * leaves;
+            }
+            gt_total++;
+            wsync();
+        }
+    }
+
+    // ---- clique levels ----------------------------------------------------------
+    DEV void clique_root_level(int32_t v) {
+        Lvl &L = ws.lv[0];
+        L.uid = const_cast<int32_t *>(p.adj + p.ostart[v]);
+        L.nu = (int)(p.off[v + 1] - p.ostart[v]);
+        L.vtx = v;
+        L.mark_lds = lds_top;
+        L.mark_glb = g_top;
+    }
+
+    DEV const int32_t *onbr(int32_t v) const { return p.adj + p.ostart[v]; }
+    DEV int odeg(int32_t v) const { return (int)(p.off[v + 1] - p.ostart[v]); }
+
+    // Intersection of two sorted lists: stride the shorter one, binary-search
+    // the longer.  WRITE: compact survivors (sorted) into out and return the
+    // count; otherwise return this lane's partial count.
+    template <bool WRITE>
+    DEV int intersect(const int32_t *A, int na, const int32_t *B, int nb, int32_t *out) {
+        const int32_t *S = A, *L = B;
+        int ns = na, nl = nb;
+        if (na > nb) {
+            S = B; L = A; ns = nb; nl = na;
+        }
+        if (ns == 0 || nl == 0) return 0;
+        if (S[ns - 1] < L[0] || L[nl - 1] < S[0]) return 0;
+        int w = 0;
+        for (int base = 0; base < ns; base += W) {
+            int t = base + g.gl;
+            bool in = t < ns;
+            bool found = false;
+            int32_t x = 0;
+            if (in) {
+                x = S[t];
+                int pos = lower_bound(L, nl, x);
+                found = pos < nl && L[pos] == x;
+            }
+            if constexpr (WRITE) {
+                uint64_t m = g.ballot(found);
+                if (found) out[w + g.below(m)] = x;
+                w += __popcll(m);
+            } else {
+                w += found ? 1 : 0;
+            }
+        }
+        return w;
+    }
+
+    DEV void clique_forward(int l, int i) {
+        Lvl &P = ws.lv[l];
+        Lvl &N = ws.lv[l + 1];
+        const int32_t c = P.uid[i];
+        const int32_t *B = P.uid + i + 1;
+        const int nb = P.nu - i - 1;
+        const int na = odeg(c);
+        N.mark_lds = lds_top;
+        N.mark_glb = g_top;
+        const int cap = na < nb ? na : nb;
+        int32_t *out = (int32_t *)alloc((int64_t)(cap > 0 ? cap : 1) * 4);
+        if (abort) return;
+        int w = intersect<true>(onbr(c), na, B, nb, out);
+        wsync();
+        N.uid = out;
+        N.nu = w;
+        N.vtx = c;
+    }
+
+    DEV void clique_leaf_phase(int l, int lo, int hi) {
+        Lvl &P = ws.lv[l];
+        for (int i = lo; i < hi; i++) {
+            if (check_abort()) return;
+            int dummy_lo = 0;
+            (void)dummy_lo;
+            // donation of this level's tail happens through lv[l].cur/end
+            P.cur = i + 1;
+            maybe_donate(l + 1, -1, dummy_lo, 0);
+            hi = P.end;
+            if (i >= hi) break;
+            const int32_t c = P.uid[i];
+            const int na = odeg(c);
+            int part = intersect<false>(onbr(c), na, P.uid + i + 1, P.nu - i - 1, nullptr);
+            cnt += (unsigned long long)part;
+            if (g.gl == 0) bytes += 4ull * (unsigned long long)na;
+        }
+    }
+
+    // ---- one job ------------------------------------------------------------
+    // children [lo, hi) of the node whose prefix ws.jid[0..lvl] is in LDS
+    DEV void run_job(int lvl, int lo, int hi) {
+        v0 = ws.jid[0];
+        job_lvl = lvl;
+        lds_top = lds_lo;
+        g_top = g_lo;
+        cnt = 0;
+        const bool motif = p.motif != 0;
+        const bool first_piece = (lvl == 0 && lo == 0);
+        if (!motif && K == 2) {
+            if (g.gl == 0) {
+                int d = odeg(v0);
+                cnt += (unsigned long long)d;
+                bytes += 4ull * (unsigned long long)d;
+            }
+            return;
+        }
+        // rebuild the prefix levels 0..lvl
+        if (motif) motif_root_level(v0);
+        else clique_root_level(v0);
+        if (first_piece && g.gl == 0)
+            bytes += 4ull * (unsigned long long)(motif ? deg(v0) : odeg(v0));
+        for (int l = 1; l <= lvl && !abort; l++) {
+            const int32_t c = ws.jid[l];
+            Lvl &P = ws.lv[l - 1];
+            int pos = lower_bound(P.uid, P.nu, c);
+            if (pos >= P.nu || P.uid[pos] != c) {
+                fail(3);
+                return;
+            }
+            if (motif) motif_forward(l - 1, c, upm_at(P, pos) & 0x7f);
+            else clique_forward(l - 1, pos);
+        }
+        if (abort) return;
+        const int leafp = K - 3;
+        {
+            Lvl &L = ws.lv[lvl];
+            int ne = motif ? L.ne : L.nu;
+            L.cur = lo < ne ? lo : ne;
+            L.end = hi < ne ? hi : ne;
+        }
+        int k = lvl;
+        while (k >= lvl) {
+            if (check_abort()) return;
+            Lvl &L = ws.lv[k];
+            if (k == leafp) {
+                if (motif) motif_leaf_phase(k, L.cur, L.end);
+                else clique_leaf_phase(k, L.cur, L.end);
+                if (abort) return;
+                lds_top = L.mark_lds;
+                g_top = L.mark_glb;
+                k--;
+                continue;
+            }
+            int dummy = 0;
+            maybe_donate(k + 1, -1, dummy, 0);
+            if (L.cur < L.end) {
+                const int i = L.cur;
+                L.cur = i + 1;
+                if (motif) {
+                    const int32_t c = L.eid[i];
+                    const int pmc = epm_at(L, i);
+                    motif_forward(k, c, pmc);
+                    if (abort) return;
+                    Lvl &N = ws.lv[k + 1];
+                    if (g.gl == 0) bytes += 4ull * (unsigned long long)N.sdeg;
+                    N.cur = 0;
+                    N.end = N.ne;
+                } else {
+                    if (g.gl == 0) bytes += 4ull * (unsigned long long)odeg(L.uid[i]);
+                    clique_forward(k, i);
+                    if (abort) return;
+                    Lvl &N = ws.lv[k + 1];
+                    N.cur = 0;
+                    N.end = N.nu;
+                }
+                k++;
+            } else {
+                lds_top = L.mark_lds;
+                g_top = L.mark_glb;
+                k--;
+            }
+        }
+    }
+};
+
+// ---------------------------------------------------------------------------
+// work acquisition.  A job head (lvl, lo, hi) comes back in registers; its
+// prefix ids go to the worker's LDS record (jid).
+struct JobHead {
+    int lvl, lo, hi;
+};
+
+template <int W>
+DEV bool take_static(const EnumParams &p, const Grp<W> &g, int w, const WorkerLDS &ws, JobHead &jh) {
+    int64_t b = p.sq_off[w], e = p.sq_off[w + 1];
+    int got = 0, r = 0, lo = 0, hi = 0;
+    if (g.gl == 0 && b < e) {
+        int cur = __hip_atomic_load(&p.sq_cur[w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (b + cur < e) {
+            int j = atomicAdd(&p.sq_cur[w], 1);
+            if (b + j < e) {
+                got = 1;
+                const int32_t *t = p.sq_jobs + 3 * (b + j);
+                r = t[0];
+                lo = t[1];
+                hi = t[2];
+                atomicSub(&p.sc->static_left, 1);
+            }
+        }
+    }
+    got = g.bcast(got, 0);
+    if (!got) return false;
+    jh.lvl = 0;
+    jh.lo = g.bcast(lo, 0);
+    jh.hi = g.bcast(hi, 0);
+    r = g.bcast(r, 0);
+    ws.jid[0] = r;
+    wsync();
+    return true;
+}
+
+template <int W>
+DEV bool pop_donated(const EnumParams &p, const Grp<W> &g, const WorkerLDS &ws, JobHead &jh, uint64_t t0) {
+    int got = 0;
+    uint32_t slot = 0;
+    if (g.gl == 0) {
+        uint32_t h = __hip_atomic_load(&p.sc->q_head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        uint32_t t = __hip_atomic_load(&p.sc->q_tail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (h < t && h < p.q_cap && atomicCAS(&p.sc->q_head, h, h + 1) == h) {
+            got = 1;
+            slot = h;
+        }
+    }
+    got = g.bcast(got, 0);
+    if (!got) return false;
+    slot = (uint32_t)g.bcast((int)slot, 0);
+    // the pusher reserved the slot before writing it: wait for the flag
+    int ready = 0;
+    while (true) {
+        if (g.gl == 0)
+            ready = __hip_atomic_load(&p.q_ready[slot], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == p.epoch;
+        ready = g.bcast(ready, 0);
+        if (ready) break;
+        if (p.watchdog_ticks && now_ticks() - t0 > p.watchdog_ticks) {
+            if (g.gl == 0) atomicCAS(&p.sc->error, 0, 1);
+            return false;
+        }
+        __builtin_amdgcn_s_sleep(1);
+    }
+    const int32_t *src = (const int32_t *)&p.q[slot];
+    jh.lvl = jh.lo = jh.hi = 0;
+    for (int j = g.gl; j < 4 + kJobIds; j += W) {
+        int v = __hip_atomic_load(src + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (j >= 4) ws.jid[j - 4] = v;
+        else if (j == 0) jh.lvl = v;
+        else if (j == 1) jh.lo = v;
+        else if (j == 2) jh.hi = v;
+    }
+    jh.lvl = g.bcast(jh.lvl, 0);
+    jh.lo = g.bcast(jh.lo, 1);
+    jh.hi = g.bcast(jh.hi, 2);
+    wsync();
+    return true;
+}
+
+template <int W>
+DEV bool steal_static(const EnumParams &p, const Grp<W> &g, int w, const WorkerLDS &ws, JobHead &jh,
+                      uint32_t &seed) {
+    int left = 0;
+    if (g.gl == 0) left = __hip_atomic_load(&p.sc->static_left, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (g.bcast(left, 0) <= 0) return false;
+    for (int probe = 0; probe < 32; probe++) {
+        seed = seed * 1664525u + 1013904223u;
+        int v = (int)((seed >> 8) % (uint32_t)p.nworkers);
+        if (v == w) continue;
+        if (take_static<W>(p, g, v, ws, jh)) return true;
+    }
+    return false;
+}
+
+template <int W>
+__global__ __launch_bounds__(kThreads) void enum_kernel(EnumParams p) {
+    extern __shared__ __align__(16) char smem[];
+    Grp<W> g;
+    const int wpb = kThreads / W;
+    const int wib = threadIdx.x / W;
+    const int wid = blockIdx.x * wpb + wib;
+    const int ncls_al = (p.ncls + 1) & ~1;
+    const size_t wsb = worker_lds_bytes(p.K);
+    unsigned long long *shist = (unsigned long long *)smem;
+    const WorkerLDS ws = worker_lds(smem + ncls_al * 8 + wib * wsb, p.K);
+    char *slice = smem + ncls_al * 8 + wpb * wsb + (int64_t)wib * p.lds_per_worker;
+    for (int i = threadIdx.x; i < p.ncls; i += kThreads) shist[i] = 0;
+    __syncthreads();
+
+    unsigned long long total = 0, bytes = 0, jobs = 0;
+    if (wid < p.nworkers) {
+        Worker<W> wk(p, ws, shist, slice, wid);
+        uint32_t seed = 0x9E3779B9u * (uint32_t)(wid + 1);
+        bool idle = false;
+        const uint64_t t0 = wk.t_start;
+        while (true) {
+            JobHead jh;
+            bool got = take_static<W>(p, g, wid, ws, jh);
+            if (!got) got = pop_donated<W>(p, g, ws, jh, t0);
+            if (!got) got = steal_static<W>(p, g, wid, ws, jh, seed);
+            if (!got) {
+                if (!idle) {
+                    if (g.gl == 0) atomicAdd(&p.sc->idle, 1);
+                    idle = true;
+                }
+                unsigned long long out = 0;
+                int err = 0;
+                if (g.gl == 0) {
+                    out = __hip_atomic_load(&p.sc->outstanding, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    err = __hip_atomic_load(&p.sc->error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                }
+                out = g.bcast64(out, 0);
+                err = g.bcast(err, 0);
+                if (out == 0 || err) break;
+                if (p.watchdog_ticks && now_ticks() - t0 > p.watchdog_ticks) {
+                    if (g.gl == 0) atomicCAS(&p.sc->error, 0, 1);
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(2);
+                continue;
+            }
+            if (idle) {
+                if (g.gl == 0) atomicSub(&p.sc->idle, 1);
+                idle = false;
+            }
+            const int32_t root = ws.jid[0];
+            if (jh.lvl < 0 || jh.lvl > (p.K == 2 ? 0 : p.K - 3) || root < 0 || root >= p.n) {
+                wk.fail(3);
+            } else {
+                const uint64_t c0 = __builtin_amdgcn_s_memtime();
+                wk.run_job(jh.lvl, jh.lo, jh.hi);
+                const uint64_t c1 = __builtin_amdgcn_s_memtime();
+                unsigned long long jc = g.lsum(wk.cnt);
+                if (g.gl == 0) {
+                    total += jc;
+                    if (p.root_cost) atomicAdd(&p.root_cost[root], (unsigned long long)(c1 - c0));
+                    if (p.root_count && jc) atomicAdd(&p.root_count[root], jc);
+                }
+            }
+            jobs++;
+            if (g.gl == 0) atomicSub(&p.sc->outstanding, 1ull);
+            if (wk.abort) break;
+        }
+        if (idle && g.gl == 0) atomicSub(&p.sc->idle, 1);
+        bytes = g.lsum(wk.bytes);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < p.ncls; i += kThreads)
+        if (shist[i]) atomicAdd(&p.hist[i], shist[i]);
+    if (wid < p.nworkers && g.gl == 0) {
+        if (total) atomicAdd(&p.sc->count, total);
+        if (bytes) atomicAdd(&p.sc->alg_bytes, bytes);
+        if (jobs) atomicAdd(&p.sc->jobs, jobs);
+    }
+}
+
+__global__ void fill_u32_kernel(uint32_t *ptr, uint32_t value, size_t count) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < count; i += stride) ptr[i] = value;
+}
+
+// ---------------------------------------------------------------------------
+size_t enum_lds_bytes(int vwave, int threads, int K, int ncls, int lds_per_worker) {
+    int wpb = threads / vwave;
+    size_t ncls_al = (size_t)((ncls + 1) & ~1);
+    return ncls_al * 8 + (size_t)wpb * worker_lds_bytes(K) + (size_t)wpb * lds_per_worker;
+}
+
+static const void *kernel_for(int vwave) {
+    switch (vwave) {
+        case 8: return (const void *)enum_kernel<8>;
+        case 16: return (const void *)enum_kernel<16>;
+        case 32: return (const void *)enum_kernel<32>;
+        case 64: return (const void *)enum_kernel<64>;
+        default: return nullptr;
+    }
+}
+
+int enum_occupancy(int vwave, int threads, size_t lds, int *blocks_per_cu) {
+    const void *k = kernel_for(vwave);
+    if (!k || threads != kThreads) return set_error(DM_ERR_ARG, "unsupported launch shape");
+    if (lds > 160 * 1024) return set_error(DM_ERR_ARG, "LDS request %zu exceeds 160 KiB", lds);
+    if (lds > 64 * 1024) {
+        hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return set_error(DM_ERR_HIP, "hipFuncSetAttribute: %s", hipGetErrorString(e));
+    }
+    int nb = 0;
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k, threads, lds);
+    if (e != hipSuccess) return set_error(DM_ERR_HIP, "occupancy query: %s", hipGetErrorString(e));
+    *blocks_per_cu = nb;
+    return DM_OK;
+}
+
+int enum_launch(const EnumParams &p, int vwave, int blocks, int threads, size_t lds, void *stream) {
+    const void *k = kernel_for(vwave);
+    if (!k || threads != kThreads) return set_error(DM_ERR_ARG, "unsupported launch shape");
+    EnumParams copy = p;
+    void *args[] = {&copy};
+    hipError_t e = hipLaunchKernel(k, dim3(blocks), dim3(threads), args, lds, (hipStream_t)stream);
+    if (e != hipSuccess) return set_error(DM_ERR_HIP, "enum launch: %s", hipGetErrorString(e));
+    return DM_OK;
+}
+
+int fill_u32_launch(void *ptr, uint32_t value, size_t count, void *stream) {
+    if (count == 0) return DM_OK;
+    int blocks = (int)((count + 255) / 256);
+    if (blocks > 1024) blocks = 1024;
+    hipLaunchKernelGGL(fill_u32_kernel, dim3(blocks), dim3(256), 0, (hipStream_t)stream,
+                       (uint32_t *)ptr, value, count);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return set_error(DM_ERR_HIP, "fill launch: %s", hipGetErrorString(e));
+    return DM_OK;
+}
+
+}  // namespace dm

@@ -1,0 +1,133 @@
+// Internal declarations shared by the host runtime (C++) and the HIP kernels.
+#ifndef DM_INTERNAL_H
+#define DM_INTERNAL_H
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/dumato.h"
+
+namespace dm {
+
+// ---- limits ---------------------------------------------------------------
+constexpr int kMaxK = 12;              // largest clique size; motifs stop at 7
+constexpr int kMaxMotifK = 7;
+constexpr int kMaxLevels = kMaxK - 2;  // stack levels 0..K-3 (+1 spare)
+constexpr int kJobIds = 12;
+
+// ---- errors ---------------------------------------------------------------
+int set_error(int code, const char *fmt, ...);
+int last_code();
+
+// ---- host graph -------------------------------------------------------------
+struct HostCSR {
+    int32_t n = 0;
+    std::vector<int64_t> off;  // n + 1
+    std::vector<int32_t> adj;  // sorted neighbour lists
+    int32_t maxdeg = 0;
+    int64_t nnz() const { return off.empty() ? 0 : off.back(); }
+};
+
+int parse_edgelist(const char *path, std::vector<int32_t> &src, std::vector<int32_t> &dst);
+int build_csr(int32_t n_hint, const int32_t *src, const int32_t *dst, size_t m, HostCSR &out);
+
+// ---- pattern tables -----------------------------------------------------------
+struct PatternTable {
+    int k = 0;
+    int ncls = 0;
+    std::vector<uint16_t> q2c;       // quick code -> class id
+    std::vector<uint64_t> cls_code;  // class id -> canonical code
+};
+const PatternTable *pattern_table(int k);
+
+// ---- random numbers (xoshiro256**, seeded by splitmix64) ---------------------
+struct Rng {
+    uint64_t s[4];
+    explicit Rng(uint64_t seed) {
+        for (int i = 0; i < 4; i++) {
+            seed += 0x9E3779B97F4A7C15ull;
+            uint64_t z = seed;
+            z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+            z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+            s[i] = z ^ (z >> 31);
+        }
+    }
+    static inline uint64_t rotl(uint64_t x, int k) { return (x << k) | (x >> (64 - k)); }
+    inline uint64_t next() {
+        uint64_t r = rotl(s[1] * 5, 7) * 9, t = s[1] << 17;
+        s[2] ^= s[0];
+        s[3] ^= s[1];
+        s[1] ^= s[2];
+        s[0] ^= s[3];
+        s[2] ^= t;
+        s[3] = rotl(s[3], 45);
+        return r;
+    }
+    inline double uniform() { return (next() >> 11) * (1.0 / 9007199254740992.0); }
+    inline uint64_t below(uint64_t n) { return (uint64_t)(uniform() * (double)n) % (n ? n : 1); }
+};
+
+// ---- device-side structures (identical layout on host and device) ----------
+struct Job {                // one unit of work: children [lo, hi) of the node
+    int32_t lvl, lo, hi, pad;  // whose prefix (ids[0..lvl]) is given
+    int32_t ids[kJobIds];
+};
+
+struct SchedCounters {
+    unsigned long long outstanding;  // unfinished jobs (static + donated)
+    unsigned long long donations;
+    unsigned long long jobs;
+    unsigned long long alg_bytes;
+    unsigned long long count;        // clique count / motif subgraph total
+    int32_t idle;                    // workers waiting for work
+    int32_t error;                   // 1 watchdog, 2 arena overflow, 3 bad job
+    uint32_t q_head, q_tail;         // donated-job queue cursors
+    int32_t static_left;
+    int32_t pad[3];
+};
+
+struct EnumParams {
+    // graph: symmetric CSR in degree-rank ids; ostart[v] = first index of N(v) above v
+    const int64_t *off;
+    const int32_t *adj;
+    const int32_t *ostart;
+    int32_t n;
+    int32_t K;
+    int32_t motif;
+    int32_t nworkers;
+    // static partition: worker w owns sq_jobs[sq_off[w] .. sq_off[w+1]) (root, lo, hi)
+    const int64_t *sq_off;
+    const int32_t *sq_jobs;
+    int32_t *sq_cur;
+    // donated jobs
+    Job *q;
+    int32_t *q_ready;
+    uint32_t q_cap;
+    int32_t epoch;
+    SchedCounters *sc;
+    // per-worker stacks: LDS slice of lds_per_worker bytes, then global arena
+    char *arena;
+    int64_t arena_per_worker;
+    int32_t lds_per_worker;
+    int32_t ncls;
+    // results
+    unsigned long long *hist;
+    const uint16_t *lut;
+    unsigned long long *root_cost;   // nullable, per degree-rank id
+    unsigned long long *root_count;  // nullable
+    int32_t steal, steal_min, steal_period, pad0;
+    uint64_t watchdog_ticks;         // wall-clock ticks
+};
+
+// kernels (enum_kernel.hip)
+size_t enum_lds_bytes(int vwave, int threads, int K, int ncls, int lds_per_worker);
+int enum_occupancy(int vwave, int threads, size_t lds, int *blocks_per_cu);
+int enum_launch(const EnumParams &p, int vwave, int blocks, int threads, size_t lds,
+                void *stream);
+int fill_u32_launch(void *ptr, uint32_t value, size_t count, void *stream);
+
+}  // namespace dm
+
+#endif

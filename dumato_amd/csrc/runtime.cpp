@@ -1,0 +1,554 @@
+// Host runtime and C ABI.
+//
+// Stands in for the reference's host class DuMatoCPU (src/main/DuMatoCPU.{h,cu}):
+// graph load -> CSR (:94-108) -> root partition (:121-135) -> device buffers
+// (:204-276) -> launch (:316-319) -> result reduction (:360-384, :419-423).
+// The host-driven stop/snapshot/donate load balancer of the reference
+// (:386-562) has no host part here: donation runs inside the kernel.
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <chrono>
+#include <climits>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+
+#include "internal.h"
+#include "runtime.h"
+
+namespace dm {
+
+static thread_local int t_code = DM_OK;
+static thread_local char t_msg[1024] = "";
+
+int set_error(int code, const char *fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(t_msg, sizeof(t_msg), fmt, ap);
+    va_end(ap);
+    t_code = code;
+    return code;
+}
+int last_code() { return t_code; }
+
+#define HIPCHK(x)                                                                        \
+    do {                                                                                 \
+        hipError_t e_ = (x);                                                             \
+        if (e_ != hipSuccess) return set_error(DM_ERR_HIP, "%s: %s", #x, hipGetErrorString(e_)); \
+    } while (0)
+
+static double now_ms() {
+    using namespace std::chrono;
+    return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
+}
+
+// ---------------------------------------------------------------------------
+int graph_finish(dm_graph *g) {
+    HostCSR &c = g->csr;
+    const int32_t n = c.n;
+    g->perm.assign(n, 0);
+    g->inv.assign(n, 0);
+    std::vector<int32_t> order(n);
+    std::iota(order.begin(), order.end(), 0);
+    std::sort(order.begin(), order.end(), [&](int32_t a, int32_t b) {
+        int64_t da = c.off[a + 1] - c.off[a], db = c.off[b + 1] - c.off[b];
+        return da != db ? da < db : a < b;
+    });
+    for (int32_t r = 0; r < n; r++) {
+        g->perm[order[r]] = r;
+        g->inv[r] = order[r];
+    }
+    HostCSR &rc = g->rcsr;
+    rc.n = n;
+    rc.maxdeg = c.maxdeg;
+    rc.off.assign((size_t)n + 1, 0);
+    rc.adj.resize(c.adj.size());
+    g->ostart.assign(n, 0);
+    int32_t max_out = 0;
+    for (int32_t r = 0; r < n; r++) {
+        int32_t v = order[r];
+        int64_t b = rc.off[r];
+        int64_t d = c.off[v + 1] - c.off[v];
+        for (int64_t i = 0; i < d; i++) rc.adj[b + i] = g->perm[c.adj[c.off[v] + i]];
+        std::sort(rc.adj.begin() + b, rc.adj.begin() + b + d);
+        rc.off[r + 1] = b + d;
+        int64_t up = std::upper_bound(rc.adj.begin() + b, rc.adj.begin() + b + d, r) - rc.adj.begin();
+        g->ostart[r] = (int32_t)up;
+        max_out = std::max<int32_t>(max_out, (int32_t)(b + d - up));
+    }
+    g->max_out = max_out;
+    return DM_OK;
+}
+
+static void device_free(DeviceGraph *d) {
+    if (!d) return;
+    auto F = [](void *p) {
+        if (p) (void)hipFree(p);
+    };
+    F(d->d_off);
+    F(d->d_adj);
+    F(d->d_ostart);
+    F(d->d_arena);
+    F(d->d_q);
+    F(d->d_q_ready);
+    F(d->d_sq_off);
+    F(d->d_sq_jobs);
+    F(d->d_sq_cur);
+    F(d->d_sc);
+    F(d->d_hist);
+    F(d->d_root_cost);
+    F(d->d_root_count);
+    for (auto &l : d->d_lut) F(l);
+    F(d->d_mlp);
+    if (d->ev0) (void)hipEventDestroy(d->ev0);
+    if (d->ev1) (void)hipEventDestroy(d->ev1);
+    if (d->stream) (void)hipStreamDestroy(d->stream);
+    delete d;
+}
+
+template <class T>
+static int grow(T *&ptr, size_t &cap, size_t need) {
+    if (need <= cap && ptr) return DM_OK;
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    cap = 0;
+    size_t alloc = std::max<size_t>(need, 1);
+    hipError_t e = hipMalloc((void **)&ptr, alloc * sizeof(T));
+    if (e != hipSuccess) return set_error(DM_ERR_NOMEM, "hipMalloc(%zu B): %s", alloc * sizeof(T), hipGetErrorString(e));
+    cap = alloc;
+    return DM_OK;
+}
+
+int ensure_device(dm_graph *g) {
+    if (g->dev) return DM_OK;
+    int ndev = 0;
+    hipError_t e = hipGetDeviceCount(&ndev);
+    if (e != hipSuccess || ndev < 1) return set_error(DM_ERR_HIP, "no HIP device available");
+    HIPCHK(hipSetDevice(0));
+    DeviceGraph *d = new DeviceGraph();
+    g->dev = d;
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, 0));
+    d->num_cu = prop.multiProcessorCount;
+    int rate = 0;
+    if (hipDeviceGetAttribute(&rate, hipDeviceAttributeWallClockRate, 0) != hipSuccess || rate <= 0) rate = 100000;
+    d->wall_khz = rate;
+    snprintf(d->arch, sizeof(d->arch), "%s", prop.gcnArchName);
+    HIPCHK(hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking));
+    HIPCHK(hipEventCreate(&d->ev0));
+    HIPCHK(hipEventCreate(&d->ev1));
+    const HostCSR &rc = g->rcsr;
+    size_t cap = 0;
+    int rc_ = grow(d->d_off, cap, (size_t)rc.n + 1);
+    if (rc_) return rc_;
+    cap = 0;
+    if ((rc_ = grow(d->d_adj, cap, std::max<size_t>(rc.adj.size(), 1)))) return rc_;
+    cap = 0;
+    if ((rc_ = grow(d->d_ostart, cap, std::max<size_t>(g->ostart.size(), 1)))) return rc_;
+    HIPCHK(hipMemcpy(d->d_off, rc.off.data(), rc.off.size() * 8, hipMemcpyHostToDevice));
+    if (!rc.adj.empty()) HIPCHK(hipMemcpy(d->d_adj, rc.adj.data(), rc.adj.size() * 4, hipMemcpyHostToDevice));
+    if (!g->ostart.empty())
+        HIPCHK(hipMemcpy(d->d_ostart, g->ostart.data(), g->ostart.size() * 4, hipMemcpyHostToDevice));
+    size_t c1 = 0;
+    if ((rc_ = grow(d->d_sc, c1, 1))) return rc_;
+    return DM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// launch shape
+static int default_lds(int vwave) { return 32768 / (256 / vwave); }
+
+int pick_shape(dm_graph *g, int K, bool motif, const dm_config &cfg, Shape &sh) {
+    DeviceGraph *d = g->dev;
+    int vw = cfg.vwave;
+    if (vw == 0) vw = autotuned_vwave(g, K, motif);
+    if (vw != 8 && vw != 16 && vw != 32 && vw != 64)
+        return set_error(DM_ERR_ARG, "vwave must be 8, 16, 32 or 64 (got %d)", vw);
+    int lds = cfg.lds_bytes > 0 ? cfg.lds_bytes : default_lds(vw);
+    lds = (lds + 15) & ~15;
+    int ncls = 0;
+    if (motif) ncls = pattern_table(K)->ncls;
+    size_t lds_total = enum_lds_bytes(vw, 256, K, ncls, lds);
+    int bpc = 0;
+    int r = enum_occupancy(vw, 256, lds_total, &bpc);
+    if (r) return r;
+    if (bpc < 1) return set_error(DM_ERR_ARG, "launch does not fit one CU (LDS %zu B)", lds_total);
+    sh.vwave = vw;
+    sh.lds_per_worker = lds;
+    sh.lds_total = lds_total;
+    sh.blocks = cfg.blocks > 0 ? cfg.blocks : bpc * d->num_cu;
+    sh.workers = sh.blocks * (256 / vw);
+    return DM_OK;
+}
+
+// per-worker global arena bound (see the allocator in enum_kernel.hip)
+static int64_t arena_bound(const dm_graph *g, int K, bool motif) {
+    int64_t md = std::max<int64_t>(g->rcsr.maxdeg, 1);
+    int64_t b = 256;
+    if (motif) {
+        for (int l = 1; l <= K - 3; l++) b += 10 * (int64_t)(l + 1) * md + 64;
+        b += 4 * md + (int64_t)(K - 2) * md + 64;
+    } else {
+        int64_t mo = std::max<int64_t>(g->max_out, 1);
+        for (int l = 1; l <= K - 3; l++) b += 4 * mo + 16;
+    }
+    return (b + 255) & ~(int64_t)255;
+}
+
+// ---------------------------------------------------------------------------
+// static partition
+bool root_has_work(const dm_graph *g, int32_t v, int K, bool motif) {
+    int64_t up = g->rcsr.off[v + 1] - g->ostart[v];
+    return motif ? up > 0 : up >= K - 1;
+}
+
+// Plain: roots sorted by degree (descending; ties by id) dealt round-robin to
+// the workers, like the reference's v -> warp v % W partition
+// (src/main/DuMatoCPU.cu:121-135) but with the heaviest roots first.
+void plain_partition(const dm_graph *g, const std::vector<int32_t> &roots, int K, bool motif,
+                     int workers, std::vector<int64_t> &sq_off, std::vector<int32_t> &sq_jobs) {
+    std::vector<int32_t> ord(roots);
+    auto wt = [&](int32_t v) -> int64_t {
+        return motif ? (g->rcsr.off[v + 1] - g->rcsr.off[v]) : (g->rcsr.off[v + 1] - g->ostart[v]);
+    };
+    std::stable_sort(ord.begin(), ord.end(), [&](int32_t a, int32_t b) { return wt(a) > wt(b); });
+    sq_off.assign((size_t)workers + 1, 0);
+    const int64_t n = (int64_t)ord.size();
+    for (int w = 0; w < workers; w++) {
+        int64_t cnt = n > w ? (n - w + workers - 1) / workers : 0;
+        sq_off[w + 1] = sq_off[w] + cnt;
+    }
+    sq_jobs.assign((size_t)n * 3, 0);
+    for (int w = 0; w < workers; w++) {
+        int64_t o = sq_off[w];
+        for (int64_t i = w; i < n; i += workers, o++) {
+            sq_jobs[3 * o] = ord[i];
+            sq_jobs[3 * o + 1] = 0;
+            sq_jobs[3 * o + 2] = INT_MAX;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+int run_enum(dm_graph *g, int K, bool motif, const dm_config &cfg, uint64_t *hist, size_t hist_len,
+             uint64_t *count_out) {
+    const double t_wall0 = now_ms();
+    const PatternTable *tab = nullptr;
+    if (motif) {
+        if (K < 3 || K > kMaxMotifK)
+            return set_error(DM_ERR_UNSUPPORTED, "motif size k=%d unsupported (3..%d)", K, kMaxMotifK);
+        tab = pattern_table(K);
+        if (!tab) return last_code();
+        if (hist && hist_len < (size_t)tab->ncls)
+            return set_error(DM_ERR_ARG, "histogram needs %d slots (got %zu)", tab->ncls, hist_len);
+    } else if (K < 2 || K > kMaxK) {
+        return set_error(DM_ERR_ARG, "clique size k=%d unsupported (2..%d)", K, kMaxK);
+    }
+    if (cfg.steal_min < 2 || cfg.steal_period < 1) return set_error(DM_ERR_ARG, "steal_min >= 2 and steal_period >= 1 required");
+    int r = ensure_device(g);
+    if (r) return r;
+    DeviceGraph *d = g->dev;
+    const int32_t n = g->rcsr.n;
+
+    Shape sh;
+    if ((r = pick_shape(g, K, motif, cfg, sh))) return r;
+
+    // roots
+    const double t_s0 = now_ms();
+    std::vector<int32_t> roots;
+    roots.reserve(n);
+    int32_t rb = std::max(cfg.root_begin, 0), re = cfg.root_end > 0 ? std::min(cfg.root_end, n) : n;
+    for (int32_t v = 0; v < n; v++) {
+        int32_t orig = g->inv[v];
+        if (orig < rb || orig >= re) continue;
+        if (root_has_work(g, v, K, motif)) roots.push_back(v);
+    }
+    std::vector<int64_t> sq_off;
+    std::vector<int32_t> sq_jobs;
+    if (cfg.sched == DM_SCHED_LEARNED) {
+        if ((r = learned_partition(g, roots, K, motif, cfg, sh, sq_off, sq_jobs))) return r;
+    } else {
+        plain_partition(g, roots, K, motif, sh.workers, sq_off, sq_jobs);
+    }
+    const double t_s1 = now_ms();
+    const int64_t njobs = (int64_t)sq_jobs.size() / 3;
+
+    // device workspaces
+    size_t need_arena = (size_t)arena_bound(g, K, motif) * (size_t)sh.workers;
+    if ((r = grow(d->d_arena, d->arena_cap, need_arena))) return r;
+    if ((r = grow(d->d_sq_off, d->sq_off_cap, sq_off.size()))) return r;
+    if ((r = grow(d->d_sq_jobs, d->sq_jobs_cap, std::max<size_t>(sq_jobs.size(), 3)))) return r;
+    if ((r = grow(d->d_sq_cur, d->sq_cur_cap, (size_t)sh.workers))) return r;
+    const uint32_t qcap = 1u << 20;
+    if ((r = grow(d->d_q, d->q_cap_alloc, qcap))) return r;
+    if (d->q_ready_cap < qcap) {
+        if ((r = grow(d->d_q_ready, d->q_ready_cap, qcap))) return r;
+        HIPCHK(hipMemset(d->d_q_ready, 0, (size_t)qcap * 4));
+        d->epoch = 0;
+    }
+    const size_t ncls = tab ? (size_t)tab->ncls : 0;
+    if ((r = grow(d->d_hist, d->hist_cap, std::max<size_t>(ncls, 1)))) return r;
+    if (tab && !d->d_lut[K]) {
+        size_t c = 0;
+        if ((r = grow(d->d_lut[K], c, tab->q2c.size()))) return r;
+        HIPCHK(hipMemcpy(d->d_lut[K], tab->q2c.data(), tab->q2c.size() * 2, hipMemcpyHostToDevice));
+    }
+    const bool want_cost = cfg.record_costs != 0, want_cnt = cfg.per_root != 0;
+    if (want_cost && (r = grow(d->d_root_cost, d->root_cost_cap, (size_t)n))) return r;
+    if (want_cnt && (r = grow(d->d_root_count, d->root_count_cap, (size_t)n))) return r;
+
+    hipStream_t s = d->stream;
+    HIPCHK(hipMemcpyAsync(d->d_sq_off, sq_off.data(), sq_off.size() * 8, hipMemcpyHostToDevice, s));
+    if (!sq_jobs.empty())
+        HIPCHK(hipMemcpyAsync(d->d_sq_jobs, sq_jobs.data(), sq_jobs.size() * 4, hipMemcpyHostToDevice, s));
+    SchedCounters sc0;
+    memset(&sc0, 0, sizeof(sc0));
+    sc0.outstanding = (unsigned long long)njobs;
+    sc0.static_left = (int32_t)std::min<int64_t>(njobs, INT_MAX);
+    HIPCHK(hipMemcpyAsync(d->d_sc, &sc0, sizeof(sc0), hipMemcpyHostToDevice, s));
+    if ((r = fill_u32_launch(d->d_sq_cur, 0, (size_t)sh.workers, s))) return r;
+    if (ncls && (r = fill_u32_launch(d->d_hist, 0, ncls * 2, s))) return r;
+    if (want_cost && (r = fill_u32_launch(d->d_root_cost, 0, (size_t)n * 2, s))) return r;
+    if (want_cnt && (r = fill_u32_launch(d->d_root_count, 0, (size_t)n * 2, s))) return r;
+
+    EnumParams p;
+    memset(&p, 0, sizeof(p));
+    p.off = d->d_off;
+    p.adj = d->d_adj;
+    p.ostart = d->d_ostart;
+    p.n = n;
+    p.K = K;
+    p.motif = motif ? 1 : 0;
+    p.nworkers = sh.workers;
+    p.sq_off = d->d_sq_off;
+    p.sq_jobs = d->d_sq_jobs;
+    p.sq_cur = d->d_sq_cur;
+    p.q = d->d_q;
+    p.q_ready = d->d_q_ready;
+    p.q_cap = qcap;
+    p.epoch = ++d->epoch;
+    p.sc = d->d_sc;
+    p.arena = (char *)d->d_arena;
+    p.arena_per_worker = arena_bound(g, K, motif);
+    p.lds_per_worker = sh.lds_per_worker;
+    p.ncls = (int32_t)ncls;
+    p.hist = d->d_hist;
+    p.lut = tab ? d->d_lut[K] : nullptr;
+    p.root_cost = want_cost ? d->d_root_cost : nullptr;
+    p.root_count = want_cnt ? d->d_root_count : nullptr;
+    p.steal = cfg.steal ? 1 : 0;
+    p.steal_min = cfg.steal_min;
+    p.steal_period = cfg.steal_period;
+    p.watchdog_ticks = cfg.watchdog_s > 0 ? (uint64_t)(cfg.watchdog_s * d->wall_khz * 1000.0) : 0;
+
+    HIPCHK(hipEventRecord(d->ev0, s));
+    if (njobs > 0 && (r = enum_launch(p, sh.vwave, sh.blocks, 256, sh.lds_total, s))) return r;
+    HIPCHK(hipEventRecord(d->ev1, s));
+    SchedCounters sc1;
+    HIPCHK(hipMemcpyAsync(&sc1, d->d_sc, sizeof(sc1), hipMemcpyDeviceToHost, s));
+    std::vector<unsigned long long> h(ncls);
+    if (ncls) HIPCHK(hipMemcpyAsync(h.data(), d->d_hist, ncls * 8, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    float kms = 0;
+    HIPCHK(hipEventElapsedTime(&kms, d->ev0, d->ev1));
+    if (sc1.error) {
+        const char *why = sc1.error == 1 ? "watchdog expired" : sc1.error == 2 ? "stack arena overflow" : "invalid job";
+        return set_error(DM_ERR_DEVICE, "enumeration kernel aborted: %s", why);
+    }
+    uint64_t total = 0;
+    if (motif) {
+        for (size_t i = 0; i < ncls; i++) total += h[i];
+        if (total != sc1.count)
+            return set_error(DM_ERR_DEVICE, "histogram sum %llu != leaf count %llu", (unsigned long long)total,
+                             (unsigned long long)sc1.count);
+        if (hist)
+            for (size_t i = 0; i < ncls; i++) hist[i] = h[i];
+    } else {
+        total = sc1.count;
+    }
+    if (count_out) *count_out = total;
+
+    // per-root records back in original ids
+    if (want_cost) {
+        std::vector<unsigned long long> tmp(n);
+        HIPCHK(hipMemcpy(tmp.data(), d->d_root_cost, (size_t)n * 8, hipMemcpyDeviceToHost));
+        g->last_costs.assign(n, 0);
+        for (int32_t v = 0; v < n; v++) g->last_costs[g->inv[v]] = tmp[v];
+    } else {
+        g->last_costs.clear();
+    }
+    if (want_cnt) {
+        std::vector<unsigned long long> tmp(n);
+        HIPCHK(hipMemcpy(tmp.data(), d->d_root_count, (size_t)n * 8, hipMemcpyDeviceToHost));
+        g->last_counts.assign(n, 0);
+        for (int32_t v = 0; v < n; v++) g->last_counts[g->inv[v]] = tmp[v];
+    } else {
+        g->last_counts.clear();
+    }
+
+    dm_stats &st = g->last;
+    memset(&st, 0, sizeof(st));
+    st.device_ms = kms;
+    st.sched_ms = t_s1 - t_s0;
+    st.wall_ms = now_ms() - t_wall0;
+    st.subgraphs = total;
+    st.alg_bytes = sc1.alg_bytes;
+    st.donations = sc1.donations;
+    st.jobs = sc1.jobs;
+    st.vwave = sh.vwave;
+    st.workers = sh.workers;
+    st.blocks = sh.blocks;
+    st.lds_bytes = sh.lds_per_worker;
+    return DM_OK;
+}
+
+}  // namespace dm
+
+using namespace dm;
+
+// ===========================================================================
+// C ABI
+extern "C" {
+
+int dm_version(void) { return 1; }
+
+const char *dm_last_error(void) { return t_msg; }
+
+int dm_config_init(dm_config *cfg) {
+    if (!cfg) return set_error(DM_ERR_ARG, "cfg is NULL");
+    memset(cfg, 0, sizeof(*cfg));
+    cfg->sched = DM_SCHED_PLAIN;
+    cfg->vwave = 0;
+    cfg->steal = 1;
+    cfg->steal_min = 2;
+    cfg->steal_period = 8;
+    cfg->lds_bytes = 0;
+    cfg->waves_per_block = 4;
+    cfg->blocks = 0;
+    cfg->watchdog_s = 120.0;
+    return DM_OK;
+}
+
+static int make_graph(std::vector<int32_t> &s, std::vector<int32_t> &d, int32_t n, dm_graph **out) {
+    if (!out) return set_error(DM_ERR_ARG, "out is NULL");
+    *out = nullptr;
+    dm_graph *g = new dm_graph();
+    int r = build_csr(n, s.data(), d.data(), s.size(), g->csr);
+    if (r == DM_OK) r = graph_finish(g);
+    if (r) {
+        delete g;
+        return r;
+    }
+    *out = g;
+    return DM_OK;
+}
+
+int dm_graph_load_edgelist(const char *path, dm_graph **out) {
+    if (!path) return set_error(DM_ERR_ARG, "path is NULL");
+    std::vector<int32_t> s, d;
+    int r = parse_edgelist(path, s, d);
+    if (r) return r;
+    return make_graph(s, d, 0, out);
+}
+
+int dm_graph_from_edges(const int32_t *src, const int32_t *dst, size_t m, int32_t n, dm_graph **out) {
+    if (m && (!src || !dst)) return set_error(DM_ERR_ARG, "edge arrays are NULL");
+    std::vector<int32_t> s(src, src + m), d(dst, dst + m);
+    return make_graph(s, d, n, out);
+}
+
+int dm_graph_info(const dm_graph *g, int32_t *n, int64_t *m, int32_t *maxdeg) {
+    if (!g) return set_error(DM_ERR_ARG, "graph is NULL");
+    if (n) *n = g->csr.n;
+    if (m) *m = g->csr.nnz() / 2;
+    if (maxdeg) *maxdeg = g->csr.maxdeg;
+    return DM_OK;
+}
+
+int dm_graph_csr(const dm_graph *g, int64_t *off, size_t off_len, int32_t *adj, size_t adj_len) {
+    if (!g) return set_error(DM_ERR_ARG, "graph is NULL");
+    if (off) {
+        if (off_len < g->csr.off.size()) return set_error(DM_ERR_ARG, "off needs %zu entries", g->csr.off.size());
+        memcpy(off, g->csr.off.data(), g->csr.off.size() * 8);
+    }
+    if (adj) {
+        if (adj_len < g->csr.adj.size()) return set_error(DM_ERR_ARG, "adj needs %zu entries", g->csr.adj.size());
+        memcpy(adj, g->csr.adj.data(), g->csr.adj.size() * 4);
+    }
+    return DM_OK;
+}
+
+void dm_graph_free(dm_graph *g) {
+    if (!g) return;
+    device_free(g->dev);
+    free_model(g);
+    delete g;
+}
+
+int dm_gen_erdos_renyi(int32_t n, int64_t m, uint64_t seed, dm_graph **out) {
+    std::vector<int32_t> s, d;
+    int r = gen_erdos_renyi(n, m, seed, s, d);
+    return r ? r : make_graph(s, d, n, out);
+}
+
+int dm_gen_chung_lu(int32_t n, int64_t m_target, double gamma, double wmax, uint64_t seed, dm_graph **out) {
+    std::vector<int32_t> s, d;
+    int r = gen_chung_lu(n, m_target, gamma, wmax, seed, s, d);
+    return r ? r : make_graph(s, d, n, out);
+}
+
+int dm_gen_rmat(int32_t scale, int32_t ef, double a, double b, double c, uint64_t seed, dm_graph **out) {
+    std::vector<int32_t> s, d;
+    int r = gen_rmat(scale, ef, a, b, c, seed, s, d);
+    return r ? r : make_graph(s, d, (int32_t)1 << scale, out);
+}
+
+int dm_gen_barabasi_albert(int32_t n, int32_t m, int32_t planted, int32_t kmin, int32_t kmax, uint64_t seed,
+                           dm_graph **out) {
+    std::vector<int32_t> s, d;
+    int r = gen_barabasi_albert(n, m, planted, kmin, kmax, seed, s, d);
+    return r ? r : make_graph(s, d, n, out);
+}
+
+int dm_clique_count(dm_graph *g, int k, const dm_config *cfg, uint64_t *out) {
+    if (!g || !out) return set_error(DM_ERR_ARG, "graph or out is NULL");
+    dm_config c;
+    if (cfg) c = *cfg;
+    else dm_config_init(&c);
+    return run_enum(g, k, false, c, nullptr, 0, out);
+}
+
+int dm_motif_count(dm_graph *g, int k, const dm_config *cfg, uint64_t *hist, size_t hist_len, uint64_t *total) {
+    if (!g || !hist) return set_error(DM_ERR_ARG, "graph or hist is NULL");
+    dm_config c;
+    if (cfg) c = *cfg;
+    else dm_config_init(&c);
+    return run_enum(g, k, true, c, hist, hist_len, total);
+}
+
+int dm_last_stats(const dm_graph *g, dm_stats *st) {
+    if (!g || !st) return set_error(DM_ERR_ARG, "NULL argument");
+    *st = g->last;
+    return DM_OK;
+}
+
+int dm_last_root_costs(const dm_graph *g, uint64_t *cycles, size_t n) {
+    if (!g || !cycles) return set_error(DM_ERR_ARG, "NULL argument");
+    if (g->last_costs.empty()) return set_error(DM_ERR_ARG, "last call did not record costs");
+    if (n < g->last_costs.size()) return set_error(DM_ERR_ARG, "need %zu entries", g->last_costs.size());
+    memcpy(cycles, g->last_costs.data(), g->last_costs.size() * 8);
+    return DM_OK;
+}
+
+int dm_last_root_counts(const dm_graph *g, uint64_t *counts, size_t n) {
+    if (!g || !counts) return set_error(DM_ERR_ARG, "NULL argument");
+    if (g->last_counts.empty()) return set_error(DM_ERR_ARG, "last call did not record per-root counts");
+    if (n < g->last_counts.size()) return set_error(DM_ERR_ARG, "need %zu entries", g->last_counts.size());
+    memcpy(counts, g->last_counts.data(), g->last_counts.size() * 8);
+    return DM_OK;
+}
+
+}  // extern "C"

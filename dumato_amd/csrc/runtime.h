@@ -1,0 +1,99 @@
+// Host-side state of a dm_graph handle.
+#ifndef DM_RUNTIME_H
+#define DM_RUNTIME_H
+
+#include <hip/hip_runtime_api.h>
+
+#include <vector>
+
+#include "internal.h"
+
+namespace dm {
+
+struct CostModel;
+
+struct DeviceGraph {
+    int num_cu = 0;
+    int wall_khz = 100000;
+    char arch[64] = "";
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    int64_t *d_off = nullptr;
+    int32_t *d_adj = nullptr;
+    int32_t *d_ostart = nullptr;
+    // grow-only workspaces
+    char *d_arena = nullptr;
+    size_t arena_cap = 0;
+    Job *d_q = nullptr;
+    size_t q_cap_alloc = 0;
+    int32_t *d_q_ready = nullptr;
+    size_t q_ready_cap = 0;
+    int64_t *d_sq_off = nullptr;
+    size_t sq_off_cap = 0;
+    int32_t *d_sq_jobs = nullptr;
+    size_t sq_jobs_cap = 0;
+    int32_t *d_sq_cur = nullptr;
+    size_t sq_cur_cap = 0;
+    SchedCounters *d_sc = nullptr;
+    unsigned long long *d_hist = nullptr;
+    size_t hist_cap = 0;
+    unsigned long long *d_root_cost = nullptr;
+    size_t root_cost_cap = 0;
+    unsigned long long *d_root_count = nullptr;
+    size_t root_count_cap = 0;
+    uint16_t *d_lut[kMaxMotifK + 1] = {nullptr};
+    float *d_mlp = nullptr;  // MLP workspace (sched.cpp)
+    size_t mlp_cap = 0;
+    int32_t epoch = 0;
+};
+
+struct Shape {
+    int vwave = 64;
+    int lds_per_worker = 0;
+    size_t lds_total = 0;
+    int blocks = 0;
+    int workers = 0;
+};
+
+}  // namespace dm
+
+struct dm_graph {
+    dm::HostCSR csr;                 // original ids
+    dm::HostCSR rcsr;                // degree-rank ids (ascending (degree, id))
+    std::vector<int32_t> perm, inv;  // perm[orig] = rank, inv[rank] = orig
+    std::vector<int32_t> ostart;     // per rank: index of the first neighbour above it
+    int32_t max_out = 0;
+    dm::DeviceGraph *dev = nullptr;
+    dm_stats last{};
+    std::vector<uint64_t> last_costs, last_counts;  // original ids
+    dm::CostModel *model = nullptr;
+    int tuned_vwave[2][dm::kMaxK + 1] = {{0}};      // [motif][k], 0 = not tuned
+};
+
+namespace dm {
+int graph_finish(dm_graph *g);
+int ensure_device(dm_graph *g);
+int pick_shape(dm_graph *g, int K, bool motif, const dm_config &cfg, Shape &sh);
+bool root_has_work(const dm_graph *g, int32_t v, int K, bool motif);
+void plain_partition(const dm_graph *g, const std::vector<int32_t> &roots, int K, bool motif, int workers,
+                     std::vector<int64_t> &sq_off, std::vector<int32_t> &sq_jobs);
+int run_enum(dm_graph *g, int K, bool motif, const dm_config &cfg, uint64_t *hist, size_t hist_len,
+             uint64_t *count_out);
+
+// sched.cpp
+int autotuned_vwave(dm_graph *g, int K, bool motif);
+int learned_partition(dm_graph *g, const std::vector<int32_t> &roots, int K, bool motif, const dm_config &cfg,
+                      const Shape &sh, std::vector<int64_t> &sq_off, std::vector<int32_t> &sq_jobs);
+void free_model(dm_graph *g);
+
+// graph.cpp
+int gen_erdos_renyi(int32_t n, int64_t m, uint64_t seed, std::vector<int32_t> &s, std::vector<int32_t> &d);
+int gen_chung_lu(int32_t n, int64_t m_target, double gamma, double wmax, uint64_t seed, std::vector<int32_t> &s,
+                 std::vector<int32_t> &d);
+int gen_rmat(int32_t scale, int32_t ef, double a, double b, double c, uint64_t seed, std::vector<int32_t> &s,
+             std::vector<int32_t> &d);
+int gen_barabasi_albert(int32_t n, int32_t m, int32_t planted, int32_t kmin, int32_t kmax, uint64_t seed,
+                        std::vector<int32_t> &s, std::vector<int32_t> &d);
+}  // namespace dm
+
+#endif

@@ -1,0 +1,141 @@
+/*
+ * dumato_amd — MI355X (gfx950) graph pattern mining runtime, C ABI.
+ *
+ * The reference (samuelbferraz/DuMato) has no library boundary: callers use
+ * its executables (clique_PA, motifs), its host class DuMatoCPU and its
+ * device API DuMatoGPU.  Each entry point below replaces one of those; the
+ * reference interface it stands in for is cited beside it.
+ *
+ * Conventions: every function returns DM_OK (0) or a negative DM_ERR_* code
+ * and never throws; dm_last_error() gives the message of the last failure on
+ * the calling thread.  Output buffers are owned by the caller.  A dm_graph
+ * handle is not thread-safe: use it from one host thread at a time.  One GPU.
+ */
+#ifndef DUMATO_AMD_H
+#define DUMATO_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DM_OK 0
+#define DM_ERR_ARG (-1)         /* bad argument (k out of range, short buffer, NULL) */
+#define DM_ERR_IO (-2)          /* edgelist cannot be opened or parsed */
+#define DM_ERR_HIP (-3)         /* HIP runtime error (no GPU, launch failure) */
+#define DM_ERR_UNSUPPORTED (-4) /* pattern size without a class table, etc. */
+#define DM_ERR_NOMEM (-5)       /* host or device allocation failed */
+#define DM_ERR_DEVICE (-6)      /* kernel reported overflow or watchdog expiry */
+
+#define DM_SCHED_PLAIN 0   /* degree-sorted static partition + work stealing */
+#define DM_SCHED_LEARNED 1 /* MLP cost-model ordering/binning + work stealing */
+
+typedef struct dm_graph dm_graph;
+
+/* Launch and scheduling knobs.  The reference's equivalents are the argv of
+ * clique_PA / motifs (threads, blockSize, donationsPerWarp, threshold;
+ * src/main/clique_PA.cu:35-60) and the hard-coded virtual-warp width of
+ * DM_WCV (src/DM_WCV/motifs_DM_WCV8.cu:65). */
+typedef struct dm_config {
+    int32_t sched;           /* DM_SCHED_PLAIN or DM_SCHED_LEARNED */
+    int32_t vwave;           /* lanes per DFS worker: 8, 16, 32, 64; 0 = autotuned */
+    int32_t steal;           /* 1: idle workers receive donated subtrees */
+    int32_t steal_min;       /* donate only levels with >= steal_min children left */
+    int32_t steal_period;    /* DFS steps between checks for idle workers */
+    int32_t lds_bytes;       /* LDS stack bytes per worker; 0 = autotuned */
+    int32_t waves_per_block; /* 0 = 4 */
+    int32_t blocks;          /* 0 = as many as stay resident on the GPU */
+    int32_t record_costs;    /* 1: per-root cycle counts (dm_last_root_costs) */
+    int32_t per_root;        /* 1: per-root subgraph counts (dm_last_root_counts) */
+    int32_t root_begin;      /* enumerate roots [root_begin, root_end) only... */
+    int32_t root_end;        /* ...(original vertex ids); root_end <= 0: all */
+    double watchdog_s;       /* device-side time limit per launch (seconds) */
+} dm_config;
+
+typedef struct dm_stats {
+    double device_ms;         /* enumeration kernel time (HIP events) */
+    double wall_ms;           /* host time of the whole call */
+    double sched_ms;          /* learned scheduling (features + inference + binning) */
+    uint64_t subgraphs;       /* count returned by the call */
+    uint64_t alg_bytes;       /* algorithmic bytes of the enumeration */
+    uint64_t donations;       /* subtrees donated to idle workers */
+    uint64_t jobs;            /* jobs executed (static + donated) */
+    int32_t vwave, workers, blocks, lds_bytes;
+} dm_stats;
+
+/* Fills *cfg with the defaults (plain scheduler, autotuned width/LDS). */
+int dm_config_init(dm_config *cfg);
+
+/* ---- graphs ---- */
+/* Reads "src dst" lines (reference: Graph::Graph, src/main/Graph.cpp:17-54):
+ * symmetric, de-duplicated, ids 0..maxId, isolated ids kept; self-loops are
+ * dropped (they change no clique or motif count). */
+int dm_graph_load_edgelist(const char *path, dm_graph **out);
+/* Same from arrays; n = max(n, max id + 1). */
+int dm_graph_from_edges(const int32_t *src, const int32_t *dst, size_t m,
+                        int32_t n, dm_graph **out);
+/* Vertex count, undirected edge count, maximum degree
+ * (reference: Graph::getNumberOfVertices/getNumberOfEdges/getMaxDegree). */
+int dm_graph_info(const dm_graph *g, int32_t *n, int64_t *m, int32_t *maxdeg);
+/* Copies the symmetric CSR in original ids: off[n+1], adj[2m]. */
+int dm_graph_csr(const dm_graph *g, int64_t *off, size_t off_len,
+                 int32_t *adj, size_t adj_len);
+void dm_graph_free(dm_graph *g);
+
+/* Seeded synthetic generators (no reference counterpart). */
+int dm_gen_erdos_renyi(int32_t n, int64_t m, uint64_t seed, dm_graph **out);
+int dm_gen_chung_lu(int32_t n, int64_t m_target, double gamma, double wmax,
+                    uint64_t seed, dm_graph **out);
+int dm_gen_rmat(int32_t scale, int32_t edge_factor, double a, double b,
+                double c, uint64_t seed, dm_graph **out);
+int dm_gen_barabasi_albert(int32_t n, int32_t m_per_vertex,
+                           int32_t planted, int32_t kmin, int32_t kmax,
+                           uint64_t seed, dm_graph **out);
+
+/* ---- counting (the hot path) ---- */
+/* Number of k-cliques, 2 <= k <= 12.  Replaces clique_PA's kernel + host
+ * loop (src/main/clique_PA.cu:9-32, DuMatoCPU::runKernel/copyResult/
+ * outputAggregateCounter, src/main/DuMatoCPU.cu:316-362,419-423). */
+int dm_clique_count(dm_graph *g, int k, const dm_config *cfg, uint64_t *out);
+
+/* Histogram of connected induced k-vertex subgraphs, 3 <= k <= 6, slot i =
+ * the reference's class id cgLocal i (dictionaries/k.csv); hist_len must be
+ * dm_motif_num_classes(k).  Replaces motifs' kernel + host loop
+ * (src/main/motifs_DM_WCLB.cu:9-36, DuMatoCPU::outputAggregatePattern,
+ * src/main/DuMatoCPU.cu:364-384).  *total (nullable) = sum of hist. */
+int dm_motif_count(dm_graph *g, int k, const dm_config *cfg, uint64_t *hist,
+                   size_t hist_len, uint64_t *total);
+
+/* Statistics and optional per-root records of the last counting call on g.
+ * Root arrays are indexed by original vertex id (n entries). */
+int dm_last_stats(const dm_graph *g, dm_stats *st);
+int dm_last_root_costs(const dm_graph *g, uint64_t *cycles, size_t n);
+int dm_last_root_counts(const dm_graph *g, uint64_t *counts, size_t n);
+
+/* ---- pattern tables (reference: QuickMapping, src/main/QuickMapping.cpp:5-43) ---- */
+/* Number of classes for k (3, 10, 33, 155 for k = 3..6). */
+int dm_motif_num_classes(int k, size_t *ncls);
+/* Regenerated tables: quick code -> class id (2^((k+1)(k-2)/2) entries) and
+ * class id -> canonical lower-triangle code (the reference's cg_global). */
+int dm_pattern_table(int k, uint16_t *quick_to_cls, size_t nquick,
+                     uint64_t *cls_to_code, size_t ncls);
+
+/* ---- learned scheduler (no reference counterpart; the reference analogue is
+ * the static partition of DuMatoCPU, src/main/DuMatoCPU.cu:121-135) ---- */
+/* Records per-root cycle costs of a plain k_train run on the GPU and trains the
+ * 8-64-64-1 MLP cost model on them with the library's HIP kernels.  The model
+ * is kept in g and used by later calls with sched = DM_SCHED_LEARNED. */
+int dm_sched_train(dm_graph *g, int mode_motif, int k_train, int epochs,
+                   uint64_t seed, double *final_loss);
+/* Predicted cost (model output, log-cycles) per original vertex id. */
+int dm_sched_predict(dm_graph *g, float *pred, size_t n);
+
+const char *dm_last_error(void);
+int dm_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
