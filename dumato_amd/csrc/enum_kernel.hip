@@ -535,15 +535,16 @@ struct Worker {
     }
 
     // ---- one job ------------------------------------------------------------
-    // children [lo, hi) of the node whose prefix ws.jid[0..lvl] is in LDS
-    DEV void run_job(int lvl, int lo, int hi) {
+    // children [lo, hi) of the node whose prefix ws.jid[0..lvl] is in LDS;
+    // first_piece: a static job that starts at the root's first child (the
+    // root node's own bytes are counted once, by that job)
+    DEV void run_job(int lvl, int lo, int hi, bool first_piece) {
         v0 = ws.jid[0];
         job_lvl = lvl;
         lds_top = lds_lo;
         g_top = g_lo;
         cnt = 0;
         const bool motif = p.motif != 0;
-        const bool first_piece = (lvl == 0 && lo == 0);
         if (!motif && K == 2) {
             if (g.gl == 0) {
                 int d = odeg(v0);
@@ -626,6 +627,7 @@ struct Worker {
 // prefix ids go to the worker's LDS record (jid).
 struct JobHead {
     int lvl, lo, hi;
+    bool donated;
 };
 
 template <int W>
@@ -649,6 +651,7 @@ DEV bool take_static(const EnumParams &p, const Grp<W> &g, int w, const WorkerLD
     got = g.bcast(got, 0);
     if (!got) return false;
     jh.lvl = 0;
+    jh.donated = false;
     jh.lo = g.bcast(lo, 0);
     jh.hi = g.bcast(hi, 0);
     r = g.bcast(r, 0);
@@ -697,6 +700,7 @@ DEV bool pop_donated(const EnumParams &p, const Grp<W> &g, const WorkerLDS &ws, 
     jh.lvl = g.bcast(jh.lvl, 0);
     jh.lo = g.bcast(jh.lo, 1);
     jh.hi = g.bcast(jh.hi, 2);
+    jh.donated = true;
     wsync();
     return true;
 }
@@ -772,7 +776,7 @@ __global__ __launch_bounds__(kThreads) void enum_kernel(EnumParams p) {
                 wk.fail(3);
             } else {
                 const uint64_t c0 = __builtin_amdgcn_s_memtime();
-                wk.run_job(jh.lvl, jh.lo, jh.hi);
+                wk.run_job(jh.lvl, jh.lo, jh.hi, !jh.donated && jh.lvl == 0 && jh.lo == 0);
                 const uint64_t c1 = __builtin_amdgcn_s_memtime();
                 unsigned long long jc = g.lsum(wk.cnt);
                 if (g.gl == 0) {
