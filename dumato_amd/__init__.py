@@ -33,7 +33,8 @@ class Config(ctypes.Structure):
                 ("lds_bytes", ctypes.c_int32), ("waves_per_block", ctypes.c_int32),
                 ("blocks", ctypes.c_int32), ("record_costs", ctypes.c_int32),
                 ("per_root", ctypes.c_int32), ("root_begin", ctypes.c_int32),
-                ("root_end", ctypes.c_int32), ("watchdog_s", ctypes.c_double)]
+                ("root_end", ctypes.c_int32), ("clique_bitmap", ctypes.c_int32),
+                ("reserved", ctypes.c_int32), ("watchdog_s", ctypes.c_double)]
 
 
 class Stats(ctypes.Structure):
@@ -84,6 +85,8 @@ SYMBOLS = [
     ("dm_sched_train", ctypes.c_int, [GP, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_uint64,
                                       ctypes.POINTER(ctypes.c_double)]),
     ("dm_sched_predict", ctypes.c_int, [GP, f32p, ctypes.c_size_t]),
+    ("dm_mlp_grad", ctypes.c_int, [f32p, f32p, f32p, ctypes.c_int, f32p, f32p, f32p]),
+    ("dm_mlp_adam", ctypes.c_int, [f32p, f32p, f32p, ctypes.c_int, ctypes.c_int, ctypes.c_float, f32p]),
 ]
 
 
@@ -244,3 +247,31 @@ def pattern_table(k):
     codes = np.zeros(ncls, np.uint64)
     _check(lib().dm_pattern_table(int(k), _ptr(q2c, u16p), nq, _ptr(codes, u64p), ncls))
     return q2c, codes
+
+
+MLP_PARAMS = 4801
+
+
+def mlp_grad(params, X, Y):
+    """(loss, grad[4801], outputs[n]) of the library's MLP kernels."""
+    P = np.ascontiguousarray(params, np.float32)
+    X = np.ascontiguousarray(X, np.float32)
+    Y = np.ascontiguousarray(Y, np.float32)
+    n = len(Y)
+    out = np.zeros(n, np.float32)
+    grad = np.zeros(MLP_PARAMS, np.float32)
+    loss = np.zeros(1, np.float32)
+    _check(lib().dm_mlp_grad(_ptr(P, f32p), _ptr(X, f32p), _ptr(Y, f32p), n, _ptr(out, f32p),
+                             _ptr(grad, f32p), _ptr(loss, f32p)))
+    return float(loss[0]), grad, out
+
+
+def mlp_adam(params, X, Y, steps, lr):
+    """Parameters after `steps` full-batch Adam steps, and the losses before each."""
+    P = np.array(params, np.float32)
+    X = np.ascontiguousarray(X, np.float32)
+    Y = np.ascontiguousarray(Y, np.float32)
+    losses = np.zeros(steps, np.float32)
+    _check(lib().dm_mlp_adam(_ptr(P, f32p), _ptr(X, f32p), _ptr(Y, f32p), len(Y), int(steps),
+                             float(lr), _ptr(losses, f32p)))
+    return P, losses

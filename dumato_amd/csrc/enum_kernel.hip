@@ -71,6 +71,14 @@ struct Grp {
         return x;
     }
     DEV int bcast(int x, int src) const { return __shfl(x, src, W); }
+    DEV int iscan(int x) const {  // inclusive prefix sum over the worker's lanes
+#pragma unroll
+        for (int o = 1; o < W; o <<= 1) {
+            int y = __shfl_up(x, o, W);
+            if (gl >= o) x += y;
+        }
+        return x;
+    }
     DEV unsigned long long bcast64(unsigned long long x, int src) const {
         return __shfl(x, src, W);
     }
@@ -211,6 +219,9 @@ struct Worker {
     }
 
     // ---- donation --------------------------------------------------------------
+    // Publish a donated job without fences: each of its 16 words is one
+    // 64-bit agent-scope atomic store tagged with the launch epoch in the
+    // high half; a reader accepts the slot once all 16 tags match.
     DEV bool push_job(int lvl, int lo, int hi) {
         int ok = 0;
         uint32_t slot = 0;
@@ -226,20 +237,17 @@ struct Worker {
         ok = g.bcast(ok, 0);
         if (!ok) return false;
         slot = (uint32_t)g.bcast((int)slot, 0);
-        int32_t *dst = (int32_t *)&p.q[slot];
-        // lanes 0..15 write the 16 words of the job
-        for (int j = g.gl; j < 16; j += W) {
+        uint64_t *dst = p.q[slot].w;
+        const uint64_t tag = (uint64_t)(uint32_t)p.epoch << 32;
+        for (int j = g.gl; j < kJobWords; j += W) {
             int32_t v;
             if (j == 0) v = lvl;
             else if (j == 1) v = lo;
             else if (j == 2) v = hi;
             else if (j == 3) v = 0;
             else v = (j - 4 <= lvl) ? ws.lv[j - 4].vtx : -1;
-            __hip_atomic_store(dst + j, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(dst + j, tag | (uint32_t)v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
-        __threadfence();
-        if (g.gl == 0)
-            __hip_atomic_store(&p.q_ready[slot], p.epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         return true;
     }
 
@@ -534,6 +542,146 @@ struct Worker {
         }
     }
 
+    // ---- clique, local bitmap of the root's out-neighbourhood -------------
+    // L = N+(v), d = |L| <= 64*NW.  rows[i*NW + w] bit b: L[64w+b] in N+(L[i]).
+    // Each lane takes first-level children i and walks the deeper levels as
+    // AND/popcount over rows; the loops visit exactly the DFS children of
+    // every node above the last level, so the algorithmic bytes (sum of
+    // out-degrees of the children of nodes with 1..k-2 vertices) are summed
+    // in the same loops from dp[] = out-degrees.
+    template <int NW>
+    DEV void bitmap_lane(const uint64_t *rows, const int32_t *dp, int i, unsigned long long &c,
+                         unsigned long long &by) const {
+        uint64_t S1[NW];
+#pragma unroll
+        for (int w = 0; w < NW; w++) S1[w] = rows[i * NW + w];
+#pragma unroll
+        for (int w1 = 0; w1 < NW; w1++) {
+            uint64_t m1 = S1[w1];
+            while (m1) {
+                const int j = w1 * 64 + __builtin_ctzll(m1);
+                m1 &= m1 - 1;
+                if (K >= 5) by += (unsigned long long)dp[j];
+                if (K == 4) {
+                    int pc = 0;
+#pragma unroll
+                    for (int w = 0; w < NW; w++) pc += __popcll(S1[w] & rows[j * NW + w]);
+                    c += (unsigned long long)pc;
+                    continue;
+                }
+                uint64_t S2[NW];
+#pragma unroll
+                for (int w = 0; w < NW; w++) S2[w] = S1[w] & rows[j * NW + w];
+#pragma unroll
+                for (int w2 = 0; w2 < NW; w2++) {
+                    uint64_t m2 = S2[w2];
+                    while (m2) {
+                        const int k = w2 * 64 + __builtin_ctzll(m2);
+                        m2 &= m2 - 1;
+                        if (K == 5) {
+                            int pc = 0;
+#pragma unroll
+                            for (int w = 0; w < NW; w++) pc += __popcll(S2[w] & rows[k * NW + w]);
+                            c += (unsigned long long)pc;
+                            continue;
+                        }
+                        by += (unsigned long long)dp[k];
+                        uint64_t S3[NW];
+#pragma unroll
+                        for (int w = 0; w < NW; w++) S3[w] = S2[w] & rows[k * NW + w];
+                        int pc = 0;
+#pragma unroll
+                        for (int w3 = 0; w3 < NW; w3++) {
+                            uint64_t m3 = S3[w3];
+                            while (m3) {
+                                const int q = w3 * 64 + __builtin_ctzll(m3);
+                                m3 &= m3 - 1;
+#pragma unroll
+                                for (int w = 0; w < NW; w++) pc += __popcll(S3[w] & rows[q * NW + w]);
+                            }
+                        }
+                        c += (unsigned long long)pc;
+                    }
+                }
+            }
+        }
+    }
+
+    template <int NW>
+    DEV void clique_bitmap_root(int32_t v, int lo, int hi, bool first_piece) {
+        const int d = odeg(v);
+        const int32_t *L = onbr(v);
+        Lvl &R = ws.lv[0];
+        R.uid = const_cast<int32_t *>(L);
+        R.nu = d;
+        R.vtx = v;
+        R.mark_lds = lds_top;
+        R.mark_glb = g_top;
+        if (hi > d) hi = d;
+        if (lo >= hi) return;
+        uint64_t *rows = (uint64_t *)alloc((int64_t)d * NW * 8);
+        int32_t *dp = (int32_t *)alloc((int64_t)d * 4);
+        int32_t *lc = (int32_t *)alloc((int64_t)d * 4);
+        int32_t *pre = (int32_t *)alloc((int64_t)(d + 1) * 4);
+        if (abort) return;
+        for (int i = g.gl; i < d; i += W) {
+            int32_t u = L[i];
+            lc[i] = u;
+            dp[i] = odeg(u);
+        }
+        for (int t = lo * NW + g.gl; t < d * NW; t += W) rows[t] = 0;
+        wsync();
+        // flatten (i, x in N+(L[i])) for i in [lo, d) and set the bits
+        int carry = 0;
+        for (int base = lo; base < d; base += W) {
+            const int i = base + g.gl;
+            const int x = i < d ? dp[i] : 0;
+            const int sc = g.iscan(x);
+            if (i < d) pre[i - lo + 1] = carry + sc;
+            carry += g.bcast(sc, W - 1);
+        }
+        if (g.gl == 0) pre[0] = 0;
+        wsync();
+        const int nrows = d - lo;
+        const int T = carry;
+        for (int t = g.gl; t < T; t += W) {
+            // row r: pre[r] <= t < pre[r+1]
+            int a = 0, b = nrows;
+            while (b - a > 1) {
+                int mid = (a + b) >> 1;
+                if (pre[mid] <= t) a = mid;
+                else b = mid;
+            }
+            const int i = lo + a;
+            const int32_t x = p.adj[p.ostart[lc[i]] + (t - pre[a])];
+            const int j = lower_bound(lc, d, x);
+            if (j < d && lc[j] == x) atomicOr(&rows[i * NW + (j >> 6)], 1ull << (j & 63));
+        }
+        wsync();
+        if (first_piece && g.gl == 0) bytes += 4ull * (unsigned long long)d;
+        // first-level children [lo, hi): one per lane, in batches of W
+        R.cur = lo;
+        R.end = hi;
+        unsigned long long by = 0;
+        while (true) {
+            if (check_abort()) break;
+            int dummy = 0;
+            maybe_donate(1, -1, dummy, 0);
+            const int cur = R.cur, end = R.end;
+            if (cur >= end) break;
+            const int take = end - cur < W ? end - cur : W;
+            R.cur = cur + take;
+            if (g.gl < take) {
+                const int i = cur + g.gl;
+                by += (unsigned long long)dp[i];
+                bitmap_lane<NW>(rows, dp, i, cnt, by);
+            }
+        }
+        bytes += 4ull * by;
+        lds_top = R.mark_lds;
+        g_top = R.mark_glb;
+    }
+
     // ---- one job ------------------------------------------------------------
     // children [lo, hi) of the node whose prefix ws.jid[0..lvl] is in LDS;
     // first_piece: a static job that starts at the root's first child (the
@@ -552,6 +700,21 @@ struct Worker {
                 bytes += 4ull * (unsigned long long)d;
             }
             return;
+        }
+        if (!motif && p.bitmap && lvl == 0 && K >= 4 && K <= 6) {
+            const int d = odeg(v0);
+            if (d <= 64) {
+                clique_bitmap_root<1>(v0, lo, hi, first_piece);
+                return;
+            }
+            if (d <= 128) {
+                clique_bitmap_root<2>(v0, lo, hi, first_piece);
+                return;
+            }
+            if (d <= 256) {
+                clique_bitmap_root<4>(v0, lo, hi, first_piece);
+                return;
+            }
         }
         // rebuild the prefix levels 0..lvl
         if (motif) motif_root_level(v0);
@@ -675,27 +838,31 @@ DEV bool pop_donated(const EnumParams &p, const Grp<W> &g, const WorkerLDS &ws, 
     got = g.bcast(got, 0);
     if (!got) return false;
     slot = (uint32_t)g.bcast((int)slot, 0);
-    // the pusher reserved the slot before writing it: wait for the flag
-    int ready = 0;
+    // the pusher reserved the slot before writing it: wait until every word
+    // carries this launch's epoch
+    const uint64_t *src = p.q[slot].w;
+    const uint32_t ep = (uint32_t)p.epoch;
+    jh.lvl = jh.lo = jh.hi = 0;
     while (true) {
-        if (g.gl == 0)
-            ready = __hip_atomic_load(&p.q_ready[slot], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == p.epoch;
-        ready = g.bcast(ready, 0);
-        if (ready) break;
+        bool mine_ok = true;
+        for (int j = g.gl; j < kJobWords; j += W) {
+            uint64_t v = __hip_atomic_load(src + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if ((uint32_t)(v >> 32) != ep) {
+                mine_ok = false;
+                continue;
+            }
+            int32_t x = (int32_t)(uint32_t)v;
+            if (j >= 4) ws.jid[j - 4] = x;
+            else if (j == 0) jh.lvl = x;
+            else if (j == 1) jh.lo = x;
+            else if (j == 2) jh.hi = x;
+        }
+        if (g.ballot(!mine_ok) == 0) break;
         if (p.watchdog_ticks && now_ticks() - t0 > p.watchdog_ticks) {
             if (g.gl == 0) atomicCAS(&p.sc->error, 0, 1);
             return false;
         }
         __builtin_amdgcn_s_sleep(1);
-    }
-    const int32_t *src = (const int32_t *)&p.q[slot];
-    jh.lvl = jh.lo = jh.hi = 0;
-    for (int j = g.gl; j < 4 + kJobIds; j += W) {
-        int v = __hip_atomic_load(src + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (j >= 4) ws.jid[j - 4] = v;
-        else if (j == 0) jh.lvl = v;
-        else if (j == 1) jh.lo = v;
-        else if (j == 2) jh.hi = v;
     }
     jh.lvl = g.bcast(jh.lvl, 0);
     jh.lo = g.bcast(jh.lo, 1);
@@ -711,7 +878,7 @@ DEV bool steal_static(const EnumParams &p, const Grp<W> &g, int w, const WorkerL
     int left = 0;
     if (g.gl == 0) left = __hip_atomic_load(&p.sc->static_left, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (g.bcast(left, 0) <= 0) return false;
-    for (int probe = 0; probe < 32; probe++) {
+    for (int probe = 0; probe < 8; probe++) {
         seed = seed * 1664525u + 1013904223u;
         int v = (int)((seed >> 8) % (uint32_t)p.nworkers);
         if (v == w) continue;
@@ -740,6 +907,7 @@ __global__ __launch_bounds__(kThreads) void enum_kernel(EnumParams p) {
         Worker<W> wk(p, ws, shist, slice, wid);
         uint32_t seed = 0x9E3779B9u * (uint32_t)(wid + 1);
         bool idle = false;
+        int backoff = 1;
         const uint64_t t0 = wk.t_start;
         while (true) {
             JobHead jh;
@@ -764,9 +932,15 @@ __global__ __launch_bounds__(kThreads) void enum_kernel(EnumParams p) {
                     if (g.gl == 0) atomicCAS(&p.sc->error, 0, 1);
                     break;
                 }
-                __builtin_amdgcn_s_sleep(2);
+                // sleep only when no lane of this wave has work: s_sleep
+                // stalls the whole wave, including busy narrower workers
+                if (__ballot(1) == ~0ull) {
+                    for (int b = 0; b < backoff; b++) __builtin_amdgcn_s_sleep(8);
+                    backoff = backoff < 32 ? backoff * 2 : 32;
+                }
                 continue;
             }
+            backoff = 1;
             if (idle) {
                 if (g.gl == 0) atomicSub(&p.sc->idle, 1);
                 idle = false;

@@ -70,9 +70,12 @@ struct Rng {
 };
 
 // ---- device-side structures (identical layout on host and device) ----------
-struct Job {                // one unit of work: children [lo, hi) of the node
-    int32_t lvl, lo, hi, pad;  // whose prefix (ids[0..lvl]) is given
-    int32_t ids[kJobIds];
+// A donated job: children [lo, hi) of the node with prefix ids[0..lvl].
+// Word j holds (launch epoch << 32 | value): words 0..3 = lvl, lo, hi, 0;
+// words 4.. = ids.
+constexpr int kJobWords = 4 + kJobIds;
+struct Job {
+    uint64_t w[kJobWords];
 };
 
 struct SchedCounters {
@@ -103,7 +106,6 @@ struct EnumParams {
     int32_t *sq_cur;
     // donated jobs
     Job *q;
-    int32_t *q_ready;
     uint32_t q_cap;
     int32_t epoch;
     SchedCounters *sc;
@@ -117,7 +119,7 @@ struct EnumParams {
     const uint16_t *lut;
     unsigned long long *root_cost;   // nullable, per degree-rank id
     unsigned long long *root_count;  // nullable
-    int32_t steal, steal_min, steal_period, pad0;
+    int32_t steal, steal_min, steal_period, bitmap;
     uint64_t watchdog_ticks;         // wall-clock ticks
 };
 

@@ -93,7 +93,6 @@ static void device_free(DeviceGraph *d) {
     F(d->d_ostart);
     F(d->d_arena);
     F(d->d_q);
-    F(d->d_q_ready);
     F(d->d_sq_off);
     F(d->d_sq_jobs);
     F(d->d_sq_cur);
@@ -194,6 +193,7 @@ static int64_t arena_bound(const dm_graph *g, int K, bool motif) {
     } else {
         int64_t mo = std::max<int64_t>(g->max_out, 1);
         for (int l = 1; l <= K - 3; l++) b += 4 * mo + 16;
+        b += 256 * 4 * 8 + 3 * 257 * 4 + 64;  // local bitmap of a root (out-degree <= 256)
     }
     return (b + 255) & ~(int64_t)255;
 }
@@ -289,10 +289,9 @@ int run_enum(dm_graph *g, int K, bool motif, const dm_config &cfg, uint64_t *his
     if ((r = grow(d->d_sq_jobs, d->sq_jobs_cap, std::max<size_t>(sq_jobs.size(), 3)))) return r;
     if ((r = grow(d->d_sq_cur, d->sq_cur_cap, (size_t)sh.workers))) return r;
     const uint32_t qcap = 1u << 20;
-    if ((r = grow(d->d_q, d->q_cap_alloc, qcap))) return r;
-    if (d->q_ready_cap < qcap) {
-        if ((r = grow(d->d_q_ready, d->q_ready_cap, qcap))) return r;
-        HIPCHK(hipMemset(d->d_q_ready, 0, (size_t)qcap * 4));
+    if (d->q_cap_alloc < qcap) {
+        if ((r = grow(d->d_q, d->q_cap_alloc, qcap))) return r;
+        HIPCHK(hipMemset(d->d_q, 0, (size_t)qcap * sizeof(Job)));  // epoch 0 never matches
         d->epoch = 0;
     }
     const size_t ncls = tab ? (size_t)tab->ncls : 0;
@@ -333,7 +332,6 @@ int run_enum(dm_graph *g, int K, bool motif, const dm_config &cfg, uint64_t *his
     p.sq_jobs = d->d_sq_jobs;
     p.sq_cur = d->d_sq_cur;
     p.q = d->d_q;
-    p.q_ready = d->d_q_ready;
     p.q_cap = qcap;
     p.epoch = ++d->epoch;
     p.sc = d->d_sc;
@@ -348,6 +346,7 @@ int run_enum(dm_graph *g, int K, bool motif, const dm_config &cfg, uint64_t *his
     p.steal = cfg.steal ? 1 : 0;
     p.steal_min = cfg.steal_min;
     p.steal_period = cfg.steal_period;
+    p.bitmap = cfg.clique_bitmap ? 1 : 0;
     p.watchdog_ticks = cfg.watchdog_s > 0 ? (uint64_t)(cfg.watchdog_s * d->wall_khz * 1000.0) : 0;
 
     HIPCHK(hipEventRecord(d->ev0, s));
@@ -434,6 +433,7 @@ int dm_config_init(dm_config *cfg) {
     cfg->lds_bytes = 0;
     cfg->waves_per_block = 4;
     cfg->blocks = 0;
+    cfg->clique_bitmap = 1;
     cfg->watchdog_s = 120.0;
     return DM_OK;
 }

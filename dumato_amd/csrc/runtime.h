@@ -7,10 +7,9 @@
 #include <vector>
 
 #include "internal.h"
+#include "mlp.h"
 
 namespace dm {
-
-struct CostModel;
 
 struct DeviceGraph {
     int num_cu = 0;
@@ -26,8 +25,6 @@ struct DeviceGraph {
     size_t arena_cap = 0;
     Job *d_q = nullptr;
     size_t q_cap_alloc = 0;
-    int32_t *d_q_ready = nullptr;
-    size_t q_ready_cap = 0;
     int64_t *d_sq_off = nullptr;
     size_t sq_off_cap = 0;
     int32_t *d_sq_jobs = nullptr;
@@ -67,6 +64,7 @@ struct dm_graph {
     dm_stats last{};
     std::vector<uint64_t> last_costs, last_counts;  // original ids
     dm::CostModel *model = nullptr;
+    std::vector<float> features;                    // kMlpIn per degree-rank vertex
     int tuned_vwave[2][dm::kMaxK + 1] = {{0}};      // [motif][k], 0 = not tuned
 };
 

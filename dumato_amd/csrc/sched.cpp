@@ -1,14 +1,35 @@
-// Scheduling policies above the plain runtime: the virtual-wave width choice
-// and the learned cost-model partition.  Neither touches the counting
-// arithmetic: they only choose launch shapes and how roots are ordered,
-// split into child ranges and dealt to workers.
+// Scheduling above the plain runtime: the virtual-wave width choice and the
+// learned cost-model partition.  Nothing here changes what is counted: the
+// policies only choose launch shapes and how roots are ordered, split into
+// child ranges and dealt to workers (reference analogue: the static
+// partition of DuMatoCPU, src/main/DuMatoCPU.cu:121-135, and the donation
+// weights of EnumerationHelper::getWeight, src/main/EnumerationHelper.cpp:125-132).
+#include <hip/hip_runtime_api.h>
+
 #include <algorithm>
+#include <chrono>
+#include <climits>
 #include <cmath>
+#include <cstring>
+#include <numeric>
+#include <queue>
 
 #include "internal.h"
+#include "mlp.h"
 #include "runtime.h"
 
 namespace dm {
+
+#define HIPCHK(x)                                                                                    \
+    do {                                                                                             \
+        hipError_t e_ = (x);                                                                         \
+        if (e_ != hipSuccess) return set_error(DM_ERR_HIP, "%s: %s", #x, hipGetErrorString(e_));     \
+    } while (0)
+
+static double now_ms() {
+    using namespace std::chrono;
+    return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
+}
 
 // Width used when the caller asks for autotuning and no tuned value is
 // cached for (mode, k): a worker's lanes stride adjacency lists, so the width
@@ -24,23 +45,448 @@ int autotuned_vwave(dm_graph *g, int K, bool motif) {
     return w;
 }
 
-void free_model(dm_graph *g) { (void)g; }
+void free_model(dm_graph *g) {
+    delete g->model;
+    g->model = nullptr;
+}
 
+// ---------------------------------------------------------------------------
+// features (per degree-rank vertex)
+static void core_numbers(const HostCSR &c, std::vector<int32_t> &core) {
+    // Batagelj-Zaversnik bucket peeling
+    const int32_t n = c.n;
+    std::vector<int32_t> deg(n), bin(c.maxdeg + 2, 0), pos(n), vert(n);
+    for (int32_t v = 0; v < n; v++) {
+        deg[v] = (int32_t)(c.off[v + 1] - c.off[v]);
+        bin[deg[v]]++;
+    }
+    int32_t start = 0;
+    for (int32_t d = 0; d <= c.maxdeg; d++) {
+        int32_t num = bin[d];
+        bin[d] = start;
+        start += num;
+    }
+    for (int32_t v = 0; v < n; v++) {
+        pos[v] = bin[deg[v]];
+        vert[pos[v]] = v;
+        bin[deg[v]]++;
+    }
+    for (int32_t d = c.maxdeg; d >= 1; d--) bin[d] = bin[d - 1];
+    bin[0] = 0;
+    for (int32_t i = 0; i < n; i++) {
+        int32_t v = vert[i];
+        for (int64_t e = c.off[v]; e < c.off[v + 1]; e++) {
+            int32_t u = c.adj[e];
+            if (deg[u] > deg[v]) {
+                int32_t du = deg[u], pu = pos[u], pw = bin[du], w = vert[pw];
+                if (u != w) {
+                    pos[u] = pw;
+                    vert[pu] = w;
+                    pos[w] = pu;
+                    vert[pw] = u;
+                }
+                bin[du]++;
+                deg[u]--;
+            }
+        }
+    }
+    core.swap(deg);
+}
+
+// 8 features per root (log1p): degree, out-degree, sum of neighbour degrees,
+// sum of out-neighbour out-degrees, core number, triangles rooted at v
+// (oriented, counted by the library's own k=3 kernel), largest neighbour
+// degree, sum of out-neighbour degrees.
+static int compute_features(dm_graph *g) {
+    if (!g->features.empty()) return DM_OK;
+    const HostCSR &c = g->rcsr;
+    const int32_t n = c.n;
+    // oriented triangles per root from a plain k=3 run
+    dm_config cfg;
+    dm_config_init(&cfg);
+    cfg.per_root = 1;
+    cfg.steal = 0;
+    uint64_t tri = 0;
+    int r = run_enum(g, 3, false, cfg, nullptr, 0, &tri);
+    if (r) return r;
+    std::vector<uint64_t> tri_orig = g->last_counts;
+    std::vector<int32_t> core;
+    core_numbers(c, core);
+    g->features.assign((size_t)n * kMlpIn, 0.f);
+    for (int32_t v = 0; v < n; v++) {
+        double deg = (double)(c.off[v + 1] - c.off[v]);
+        double up = (double)(c.off[v + 1] - g->ostart[v]);
+        double s2 = 0, s2o = 0, mx = 0, s2u = 0;
+        for (int64_t e = c.off[v]; e < c.off[v + 1]; e++) {
+            int32_t u = c.adj[e];
+            double du = (double)(c.off[u + 1] - c.off[u]);
+            s2 += du;
+            mx = std::max(mx, du);
+            if (e >= g->ostart[v]) {
+                s2o += (double)(c.off[u + 1] - g->ostart[u]);
+                s2u += du;
+            }
+        }
+        float *f = &g->features[(size_t)v * kMlpIn];
+        f[0] = (float)std::log1p(deg);
+        f[1] = (float)std::log1p(up);
+        f[2] = (float)std::log1p(s2);
+        f[3] = (float)std::log1p(s2o);
+        f[4] = (float)std::log1p((double)core[v]);
+        f[5] = (float)std::log1p((double)tri_orig[g->inv[v]]);
+        f[6] = (float)std::log1p(mx);
+        f[7] = (float)std::log1p(s2u);
+    }
+    return DM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// device buffers for training / inference
+struct MlpDev {
+    float *P = nullptr, *M = nullptr, *V = nullptr, *grad = nullptr, *loss = nullptr;
+    float *X = nullptr, *Y = nullptr;
+    MlpScratch s;
+    int cap = 0;
+};
+
+static void mlp_dev_free(MlpDev &d) {
+    float *ptrs[] = {d.P, d.M, d.V, d.grad, d.loss, d.X, d.Y, d.s.out, d.s.h1, d.s.h2, d.s.dz1, d.s.dz2, d.s.g};
+    for (float *p : ptrs)
+        if (p) (void)hipFree(p);
+    d = MlpDev();
+}
+
+static int mlp_dev_alloc(MlpDev &d, int cap) {
+    auto A = [](float **p, size_t n) { return hipMalloc((void **)p, std::max<size_t>(n, 1) * sizeof(float)); };
+    hipError_t e = hipSuccess;
+    if (e == hipSuccess) e = A(&d.P, kMlpParams);
+    if (e == hipSuccess) e = A(&d.M, kMlpParams);
+    if (e == hipSuccess) e = A(&d.V, kMlpParams);
+    if (e == hipSuccess) e = A(&d.grad, kMlpParams);
+    if (e == hipSuccess) e = A(&d.loss, 1);
+    if (e == hipSuccess) e = A(&d.X, (size_t)cap * kMlpIn);
+    if (e == hipSuccess) e = A(&d.Y, (size_t)cap);
+    if (e == hipSuccess) e = A(&d.s.out, (size_t)cap);
+    if (e == hipSuccess) e = A(&d.s.g, (size_t)cap);
+    if (e == hipSuccess) e = A(&d.s.h1, (size_t)cap * kMlpHidden);
+    if (e == hipSuccess) e = A(&d.s.h2, (size_t)cap * kMlpHidden);
+    if (e == hipSuccess) e = A(&d.s.dz1, (size_t)cap * kMlpHidden);
+    if (e == hipSuccess) e = A(&d.s.dz2, (size_t)cap * kMlpHidden);
+    if (e != hipSuccess) {
+        mlp_dev_free(d);
+        return set_error(DM_ERR_NOMEM, "MLP buffers: %s", hipGetErrorString(e));
+    }
+    d.cap = cap;
+    return DM_OK;
+}
+
+static void init_params(std::vector<float> &P, uint64_t seed) {
+    Rng r(seed);
+    P.assign(kMlpParams, 0.f);
+    auto normal = [&]() {
+        double u1 = std::max(r.uniform(), 1e-12), u2 = r.uniform();
+        return std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
+    };
+    const int oW1 = 0, ob1 = kMlpIn * kMlpHidden, oW2 = ob1 + kMlpHidden, ob2 = oW2 + kMlpHidden * kMlpHidden,
+              oW3 = ob2 + kMlpHidden;
+    for (int i = 0; i < kMlpIn * kMlpHidden; i++) P[oW1 + i] = (float)(normal() * std::sqrt(2.0 / kMlpIn));
+    for (int i = 0; i < kMlpHidden * kMlpHidden; i++) P[oW2 + i] = (float)(normal() * std::sqrt(2.0 / kMlpHidden));
+    for (int i = 0; i < kMlpHidden; i++) P[oW3 + i] = (float)(normal() * std::sqrt(1.0 / kMlpHidden));
+}
+
+// Train on (X, Y) (host, standardised) with minibatch Adam on the GPU.
+static int train_model(const std::vector<float> &X, const std::vector<float> &Y, int epochs, uint64_t seed,
+                       std::vector<float> &P, double &final_loss) {
+    const int n = (int)Y.size();
+    const int batch = std::min(n, 4096);
+    MlpDev d;
+    int r = mlp_dev_alloc(d, batch);
+    if (r) return r;
+    init_params(P, seed);
+    HIPCHK(hipMemcpy(d.P, P.data(), kMlpParams * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemset(d.M, 0, kMlpParams * 4));
+    HIPCHK(hipMemset(d.V, 0, kMlpParams * 4));
+    std::vector<int> idx(n);
+    std::iota(idx.begin(), idx.end(), 0);
+    Rng rng(seed ^ 0x5bd1e995ull);
+    std::vector<float> bx((size_t)batch * kMlpIn), by(batch);
+    int step = 0;
+    double epoch_loss = 0;
+    for (int ep = 0; ep < epochs; ep++) {
+        for (int i = n - 1; i > 0; i--) std::swap(idx[i], idx[rng.below((uint64_t)i + 1)]);
+        epoch_loss = 0;
+        int nb = 0;
+        for (int s = 0; s + batch <= n || (s < n && nb == 0); s += batch) {
+            const int m = std::min(batch, n - s);
+            for (int t = 0; t < m; t++) {
+                memcpy(&bx[(size_t)t * kMlpIn], &X[(size_t)idx[s + t] * kMlpIn], kMlpIn * 4);
+                by[t] = Y[idx[s + t]];
+            }
+            HIPCHK(hipMemcpy(d.X, bx.data(), (size_t)m * kMlpIn * 4, hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(d.Y, by.data(), (size_t)m * 4, hipMemcpyHostToDevice));
+            HIPCHK(hipMemset(d.loss, 0, 4));
+            if ((r = mlp_grad_launch(d.P, d.X, d.Y, m, d.s, d.grad, d.loss, nullptr))) return r;
+            if ((r = mlp_adam_launch(d.P, d.grad, d.M, d.V, 1e-3f, 0.9f, 0.999f, 1e-8f, ++step, nullptr))) return r;
+            float l = 0;
+            HIPCHK(hipMemcpy(&l, d.loss, 4, hipMemcpyDeviceToHost));
+            epoch_loss += l;
+            nb++;
+        }
+        epoch_loss /= std::max(nb, 1);
+    }
+    HIPCHK(hipMemcpy(P.data(), d.P, kMlpParams * 4, hipMemcpyDeviceToHost));
+    mlp_dev_free(d);
+    final_loss = epoch_loss;
+    return DM_OK;
+}
+
+// Predicted log-cycles per degree-rank vertex.
+static int predict_all(dm_graph *g, std::vector<float> &pred) {
+    const CostModel &M = *g->model;
+    const int32_t n = g->rcsr.n;
+    std::vector<float> X((size_t)n * kMlpIn);
+    for (int32_t v = 0; v < n; v++)
+        for (int i = 0; i < kMlpIn; i++)
+            X[(size_t)v * kMlpIn + i] = (g->features[(size_t)v * kMlpIn + i] - M.feat_mean[i]) / M.feat_std[i];
+    float *dP = nullptr, *dX = nullptr, *dO = nullptr;
+    auto cleanup = [&]() {
+        if (dP) (void)hipFree(dP);
+        if (dX) (void)hipFree(dX);
+        if (dO) (void)hipFree(dO);
+    };
+    if (hipMalloc(&dP, kMlpParams * 4) != hipSuccess || hipMalloc(&dX, X.size() * 4 + 4) != hipSuccess ||
+        hipMalloc(&dO, (size_t)n * 4 + 4) != hipSuccess) {
+        cleanup();
+        return set_error(DM_ERR_NOMEM, "MLP inference buffers");
+    }
+    hipMemcpy(dP, M.params.data(), kMlpParams * 4, hipMemcpyHostToDevice);
+    hipMemcpy(dX, X.data(), X.size() * 4, hipMemcpyHostToDevice);
+    int r = mlp_forward_launch(dP, dX, n, dO, nullptr);
+    pred.assign(n, 0.f);
+    if (!r && hipMemcpy(pred.data(), dO, (size_t)n * 4, hipMemcpyDeviceToHost) != hipSuccess)
+        r = set_error(DM_ERR_HIP, "MLP inference copy");
+    cleanup();
+    if (r) return r;
+    for (auto &p : pred) p = p * M.y_std + M.y_mean;
+    return DM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// learned partition: order by predicted cost, split predicted-heavy roots into
+// child ranges, deal pieces to the least-loaded worker (LPT).
 int learned_partition(dm_graph *g, const std::vector<int32_t> &roots, int K, bool motif, const dm_config &cfg,
                       const Shape &sh, std::vector<int64_t> &sq_off, std::vector<int32_t> &sq_jobs) {
-    (void)g; (void)roots; (void)K; (void)motif; (void)cfg; (void)sh; (void)sq_off; (void)sq_jobs;
-    return set_error(DM_ERR_ARG, "learned scheduler: no trained model on this graph");
+    (void)cfg;
+    if (!g->model) return set_error(DM_ERR_ARG, "learned scheduler: train a model first (dm_sched_train)");
+    int r = compute_features(g);
+    if (r) return r;
+    std::vector<float> pred;
+    if ((r = predict_all(g, pred))) return r;
+    struct Piece {
+        double cost;
+        int32_t root, lo, hi;
+    };
+    std::vector<Piece> pieces;
+    pieces.reserve(roots.size() * 2);
+    double total = 0;
+    for (int32_t v : roots) total += std::exp((double)pred[v]);
+    const int workers = sh.workers;
+    const double target = total / std::max(workers, 1);
+    const bool can_split = !(K == 2 && !motif);
+    for (int32_t v : roots) {
+        const double c = std::exp((double)pred[v]);
+        const int nch = (int)(g->rcsr.off[v + 1] - g->ostart[v]);  // children of the root
+        int parts = 1;
+        if (can_split && c > 0.5 * target && nch >= 2)
+            parts = (int)std::min<double>(nch, std::ceil(c / (0.5 * target)));
+        if (parts <= 1) {
+            pieces.push_back({c, v, 0, INT_MAX});
+            continue;
+        }
+        // child i of a root carries roughly (nch - i)^e of its work (later
+        // children see fewer candidates above them)
+        const double e = motif ? 1.0 : (double)std::max(1, K - 3);
+        std::vector<double> w(nch);
+        double ws = 0;
+        for (int i = 0; i < nch; i++) ws += (w[i] = std::pow((double)(nch - i), e));
+        double acc = 0;
+        int lo = 0;
+        for (int pidx = 1; pidx <= parts && lo < nch; pidx++) {
+            const double goal = ws * pidx / parts;
+            int hi = lo;
+            double a = acc;
+            while (hi < nch && (a < goal || hi == lo)) a += w[hi++];
+            if (pidx == parts) {
+                while (hi < nch) a += w[hi++];
+            }
+            pieces.push_back({c * (a - acc) / ws, v, lo, hi >= nch ? INT_MAX : hi});
+            acc = a;
+            lo = hi;
+        }
+    }
+    std::sort(pieces.begin(), pieces.end(), [](const Piece &a, const Piece &b) { return a.cost > b.cost; });
+    using QE = std::pair<double, int>;
+    std::priority_queue<QE, std::vector<QE>, std::greater<QE>> heap;
+    for (int w = 0; w < workers; w++) heap.push({0.0, w});
+    std::vector<std::vector<int32_t>> q(workers);
+    for (const Piece &p : pieces) {
+        QE t = heap.top();
+        heap.pop();
+        q[t.second].push_back((int32_t)(&p - pieces.data()));
+        t.first += p.cost;
+        heap.push(t);
+    }
+    sq_off.assign((size_t)workers + 1, 0);
+    for (int w = 0; w < workers; w++) sq_off[w + 1] = sq_off[w] + (int64_t)q[w].size();
+    sq_jobs.assign((size_t)sq_off[workers] * 3, 0);
+    for (int w = 0; w < workers; w++) {
+        int64_t o = sq_off[w];
+        for (int32_t pi : q[w]) {
+            sq_jobs[3 * o] = pieces[pi].root;
+            sq_jobs[3 * o + 1] = pieces[pi].lo;
+            sq_jobs[3 * o + 2] = pieces[pi].hi;
+            o++;
+        }
+    }
+    return DM_OK;
+}
+
+int sched_train(dm_graph *g, int mode_motif, int k_train, int epochs, uint64_t seed, double *final_loss) {
+    if (!g) return set_error(DM_ERR_ARG, "graph is NULL");
+    if (epochs < 1) return set_error(DM_ERR_ARG, "epochs must be >= 1");
+    int r = compute_features(g);
+    if (r) return r;
+    dm_config cfg;
+    dm_config_init(&cfg);
+    cfg.record_costs = 1;
+    cfg.steal = 0;
+    const bool motif = mode_motif != 0;
+    std::vector<uint64_t> hist;
+    uint64_t tot = 0;
+    if (motif) {
+        const PatternTable *t = pattern_table(k_train);
+        if (!t) return last_code();
+        hist.assign(t->ncls, 0);
+        r = run_enum(g, k_train, true, cfg, hist.data(), hist.size(), &tot);
+    } else {
+        r = run_enum(g, k_train, false, cfg, nullptr, 0, &tot);
+    }
+    if (r) return r;
+    // samples: roots with work, target log(cycles)
+    const int32_t n = g->rcsr.n;
+    std::vector<int32_t> roots;
+    for (int32_t v = 0; v < n; v++)
+        if (g->last_costs[g->inv[v]] > 0) roots.push_back(v);
+    if (roots.size() < 16) return set_error(DM_ERR_ARG, "too few roots with work to train on");
+    CostModel *M = new CostModel();
+    M->mode_motif = mode_motif;
+    M->k_train = k_train;
+    const size_t ns = roots.size();
+    std::vector<float> X(ns * kMlpIn), Y(ns);
+    for (int i = 0; i < kMlpIn; i++) {
+        double s = 0, s2 = 0;
+        for (int32_t v : roots) {
+            double x = g->features[(size_t)v * kMlpIn + i];
+            s += x;
+            s2 += x * x;
+        }
+        double mu = s / ns, var = std::max(s2 / ns - mu * mu, 1e-12);
+        M->feat_mean[i] = (float)mu;
+        M->feat_std[i] = (float)std::sqrt(var);
+    }
+    double ys = 0, ys2 = 0;
+    for (size_t t = 0; t < ns; t++) {
+        double y = std::log((double)g->last_costs[g->inv[roots[t]]]);
+        Y[t] = (float)y;
+        ys += y;
+        ys2 += y * y;
+        for (int i = 0; i < kMlpIn; i++)
+            X[t * kMlpIn + i] = (g->features[(size_t)roots[t] * kMlpIn + i] - M->feat_mean[i]) / M->feat_std[i];
+    }
+    M->y_mean = (float)(ys / ns);
+    M->y_std = (float)std::sqrt(std::max(ys2 / ns - (ys / ns) * (ys / ns), 1e-12));
+    for (auto &y : Y) y = (y - M->y_mean) / M->y_std;
+    double loss = 0;
+    if ((r = train_model(X, Y, epochs, seed, M->params, loss))) {
+        delete M;
+        return r;
+    }
+    M->final_loss = loss;
+    free_model(g);
+    g->model = M;
+    if (final_loss) *final_loss = loss;
+    return DM_OK;
 }
 
 }  // namespace dm
 
+using namespace dm;
+
 extern "C" int dm_sched_train(dm_graph *g, int mode_motif, int k_train, int epochs, uint64_t seed,
                               double *final_loss) {
-    (void)g; (void)mode_motif; (void)k_train; (void)epochs; (void)seed; (void)final_loss;
-    return dm::set_error(DM_ERR_UNSUPPORTED, "not built yet");
+    return sched_train(g, mode_motif, k_train, epochs, seed, final_loss);
 }
 
 extern "C" int dm_sched_predict(dm_graph *g, float *pred, size_t n) {
-    (void)g; (void)pred; (void)n;
-    return dm::set_error(DM_ERR_UNSUPPORTED, "not built yet");
+    if (!g || !pred) return set_error(DM_ERR_ARG, "NULL argument");
+    if (!g->model) return set_error(DM_ERR_ARG, "no trained model");
+    if (n < (size_t)g->rcsr.n) return set_error(DM_ERR_ARG, "need %d entries", g->rcsr.n);
+    int r = ensure_device(g);
+    if (r) return r;
+    if ((r = compute_features(g))) return r;
+    std::vector<float> p;
+    if ((r = predict_all(g, p))) return r;
+    for (int32_t v = 0; v < g->rcsr.n; v++) pred[g->inv[v]] = p[v];
+    return DM_OK;
+}
+
+// Test entry points for the MLP kernels (no graph involved).
+extern "C" int dm_mlp_grad(const float *params, const float *X, const float *Y, int n, float *out, float *grad,
+                           float *loss) {
+    if (!params || !X || !Y || n < 1 || !grad || !loss) return set_error(DM_ERR_ARG, "NULL argument");
+    int nd = 0;
+    if (hipGetDeviceCount(&nd) != hipSuccess || nd < 1) return set_error(DM_ERR_HIP, "no HIP device available");
+    MlpDev d;
+    int r = mlp_dev_alloc(d, n);
+    if (r) return r;
+    hipMemcpy(d.P, params, kMlpParams * 4, hipMemcpyHostToDevice);
+    hipMemcpy(d.X, X, (size_t)n * kMlpIn * 4, hipMemcpyHostToDevice);
+    hipMemcpy(d.Y, Y, (size_t)n * 4, hipMemcpyHostToDevice);
+    hipMemset(d.loss, 0, 4);
+    r = mlp_grad_launch(d.P, d.X, d.Y, n, d.s, d.grad, d.loss, nullptr);
+    if (!r) {
+        hipMemcpy(grad, d.grad, kMlpParams * 4, hipMemcpyDeviceToHost);
+        hipMemcpy(loss, d.loss, 4, hipMemcpyDeviceToHost);
+        if (out) hipMemcpy(out, d.s.out, (size_t)n * 4, hipMemcpyDeviceToHost);
+        if (hipDeviceSynchronize() != hipSuccess) r = set_error(DM_ERR_HIP, "MLP gradient");
+    }
+    mlp_dev_free(d);
+    return r;
+}
+
+extern "C" int dm_mlp_adam(float *params, const float *X, const float *Y, int n, int steps, float lr,
+                           float *losses) {
+    if (!params || !X || !Y || n < 1 || steps < 1) return set_error(DM_ERR_ARG, "bad argument");
+    int nd = 0;
+    if (hipGetDeviceCount(&nd) != hipSuccess || nd < 1) return set_error(DM_ERR_HIP, "no HIP device available");
+    MlpDev d;
+    int r = mlp_dev_alloc(d, n);
+    if (r) return r;
+    hipMemcpy(d.P, params, kMlpParams * 4, hipMemcpyHostToDevice);
+    hipMemset(d.M, 0, kMlpParams * 4);
+    hipMemset(d.V, 0, kMlpParams * 4);
+    hipMemcpy(d.X, X, (size_t)n * kMlpIn * 4, hipMemcpyHostToDevice);
+    hipMemcpy(d.Y, Y, (size_t)n * 4, hipMemcpyHostToDevice);
+    for (int s = 0; s < steps && !r; s++) {
+        hipMemset(d.loss, 0, 4);
+        r = mlp_grad_launch(d.P, d.X, d.Y, n, d.s, d.grad, d.loss, nullptr);
+        if (!r) r = mlp_adam_launch(d.P, d.grad, d.M, d.V, lr, 0.9f, 0.999f, 1e-8f, s + 1, nullptr);
+        if (!r && losses) hipMemcpy(&losses[s], d.loss, 4, hipMemcpyDeviceToHost);
+    }
+    if (!r) {
+        hipMemcpy(params, d.P, kMlpParams * 4, hipMemcpyDeviceToHost);
+        if (hipDeviceSynchronize() != hipSuccess) r = set_error(DM_ERR_HIP, "MLP Adam");
+    }
+    mlp_dev_free(d);
+    return r;
 }

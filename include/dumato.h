@@ -51,6 +51,9 @@ typedef struct dm_config {
     int32_t per_root;        /* 1: per-root subgraph counts (dm_last_root_counts) */
     int32_t root_begin;      /* enumerate roots [root_begin, root_end) only... */
     int32_t root_end;        /* ...(original vertex ids); root_end <= 0: all */
+    int32_t clique_bitmap;   /* 1: k=4..6 cliques of roots with out-degree <= 256 count
+                                on a local adjacency bitmap; 0: list intersections */
+    int32_t reserved;
     double watchdog_s;       /* device-side time limit per launch (seconds) */
 } dm_config;
 
@@ -131,6 +134,15 @@ int dm_sched_train(dm_graph *g, int mode_motif, int k_train, int epochs,
                    uint64_t seed, double *final_loss);
 /* Predicted cost (model output, log-cycles) per original vertex id. */
 int dm_sched_predict(dm_graph *g, float *pred, size_t n);
+/* The MLP kernels on their own, for checking against a CPU reference.
+ * params: 4801 floats, W1[8][64], b1[64], W2[64][64], b2[64], W3[64], b3.
+ * dm_mlp_grad: loss = mean_b (f(X_b) - Y_b)^2, its gradient, f(X) (nullable).
+ * dm_mlp_adam: `steps` full-batch Adam steps (beta 0.9/0.999, eps 1e-8) in
+ * place on params; losses[steps] (nullable) before each step. */
+int dm_mlp_grad(const float *params, const float *X, const float *Y, int n,
+                float *out, float *grad, float *loss);
+int dm_mlp_adam(float *params, const float *X, const float *Y, int n, int steps,
+                float lr, float *losses);
 
 const char *dm_last_error(void);
 int dm_version(void);

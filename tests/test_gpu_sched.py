@@ -1,0 +1,70 @@
+"""Learned scheduler: trained on the GPU from the runtime's own per-root
+cycle counters; it may only reorder/split/bin work, so every count must equal
+the plain runtime's (and the oracle's)."""
+import numpy as np
+import pytest
+
+import dumato_amd as dm
+from oracle import pyoracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def spearman(a, b):
+    ra = np.argsort(np.argsort(a)).astype(np.float64)
+    rb = np.argsort(np.argsort(b)).astype(np.float64)
+    ra -= ra.mean()
+    rb -= rb.mean()
+    return float((ra * rb).sum() / np.sqrt((ra * ra).sum() * (rb * rb).sum()))
+
+
+@pytest.fixture(scope="module")
+def graph():
+    g = dm.Graph.chung_lu(6000, 60000, 2.2, 300.0, 77)
+    n, m, _ = g.info()
+    off, adj = g.csr()
+    return g, O.CSR(n, off, adj)
+
+
+def test_learned_clique_counts_equal_plain(graph):
+    g, ref = graph
+    print("case learned clique train", flush=True)
+    loss = g.sched_train(False, 4, epochs=20, seed=3)
+    assert np.isfinite(loss)
+    for k in (4, 5):
+        print("case learned clique k=%d" % k, flush=True)
+        want = O.clique_count(ref, k)
+        for w in (16, 64):
+            assert g.clique_count(k, dm.config(sched=dm.DM_SCHED_LEARNED, vwave=w)) == want
+            assert g.clique_count(k, dm.config(sched=dm.DM_SCHED_LEARNED, vwave=w, steal=0)) == want
+        assert g.clique_count(k, dm.config(sched=dm.DM_SCHED_LEARNED, clique_bitmap=0)) == want
+
+
+def test_learned_motif_counts_equal_plain(graph):
+    g, ref = graph
+    print("case learned motif train", flush=True)
+    g.sched_train(True, 3, epochs=20, seed=4)
+    wt, wh = O.motif_count(ref, 4)
+    for w in (16, 64):
+        print("case learned motif k=4 w=%d" % w, flush=True)
+        tot, hist = g.motif_count(4, dm.config(sched=dm.DM_SCHED_LEARNED, vwave=w))
+        assert tot == wt and np.array_equal(hist, wh)
+
+
+def test_model_ranks_costs(graph):
+    g, _ = graph
+    print("case model ranks costs", flush=True)
+    g.sched_train(False, 4, epochs=30, seed=5)
+    pred = g.sched_predict()
+    g.clique_count(4, dm.config(record_costs=1, steal=0))
+    cost = g.last_root_costs().astype(np.float64)
+    has = cost > 0
+    rho = spearman(pred[has], np.log(cost[has]))
+    print("spearman(pred, log cost) = %.3f over %d roots" % (rho, int(has.sum())), flush=True)
+    assert rho > 0.5
+
+
+def test_learned_without_model_is_an_error():
+    g = dm.Graph.erdos_renyi(200, 800, 1)
+    with pytest.raises(dm.DMError):
+        g.clique_count(3, dm.config(sched=dm.DM_SCHED_LEARNED))
