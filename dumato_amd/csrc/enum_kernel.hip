@@ -561,7 +561,7 @@ struct Worker {
             while (m1) {
                 const int j = w1 * 64 + __builtin_ctzll(m1);
                 m1 &= m1 - 1;
-                if (K >= 5) by += (unsigned long long)dp[j];
+                by += (unsigned long long)dp[j];
                 if (K == 4) {
                     int pc = 0;
 #pragma unroll
@@ -578,6 +578,7 @@ struct Worker {
                     while (m2) {
                         const int k = w2 * 64 + __builtin_ctzll(m2);
                         m2 &= m2 - 1;
+                        by += (unsigned long long)dp[k];
                         if (K == 5) {
                             int pc = 0;
 #pragma unroll
@@ -585,7 +586,6 @@ struct Worker {
                             c += (unsigned long long)pc;
                             continue;
                         }
-                        by += (unsigned long long)dp[k];
                         uint64_t S3[NW];
 #pragma unroll
                         for (int w = 0; w < NW; w++) S3[w] = S2[w] & rows[k * NW + w];
@@ -596,6 +596,7 @@ struct Worker {
                             while (m3) {
                                 const int q = w3 * 64 + __builtin_ctzll(m3);
                                 m3 &= m3 - 1;
+                                by += (unsigned long long)dp[q];
 #pragma unroll
                                 for (int w = 0; w < NW; w++) pc += __popcll(S3[w] & rows[q * NW + w]);
                             }

@@ -95,12 +95,13 @@ def test_complete_graph_closed_form():
     assert tot == math.comb(24, 5) and hist[32] == tot
 
 
-def test_per_root_counts_and_bytes(citeseer):
+@pytest.mark.parametrize("bitmap", [1, 0])
+def test_per_root_counts_and_bytes(citeseer, bitmap):
     g, ref = citeseer
     perm, ro = O.relabel(ref, oriented=True)
-    for k in (3, 4, 5):
-        print("case per-root clique k=%d" % k, flush=True)
-        tot = g.clique_count(k, cfg(per_root=1, vwave=16))
+    for k in (3, 4, 5, 6):
+        print("case per-root clique k=%d bitmap=%d" % (k, bitmap), flush=True)
+        tot = g.clique_count(k, cfg(per_root=1, vwave=16, clique_bitmap=bitmap))
         pr = g.last_root_counts()
         st = g.last_stats()
         _, opr_rel = O.clique_count(ro, k, per_root=True)
