@@ -42,7 +42,8 @@ class Stats(ctypes.Structure):
     _fields_ = [("device_ms", ctypes.c_double), ("wall_ms", ctypes.c_double),
                 ("sched_ms", ctypes.c_double), ("subgraphs", ctypes.c_uint64),
                 ("alg_bytes", ctypes.c_uint64), ("donations", ctypes.c_uint64),
-                ("jobs", ctypes.c_uint64), ("vwave", ctypes.c_int32), ("workers", ctypes.c_int32),
+                ("jobs", ctypes.c_uint64), ("busy_sum_cycles", ctypes.c_uint64),
+                ("busy_max_cycles", ctypes.c_uint64), ("vwave", ctypes.c_int32), ("workers", ctypes.c_int32),
                 ("blocks", ctypes.c_int32), ("lds_bytes", ctypes.c_int32)]
 
     def as_dict(self):
@@ -85,6 +86,7 @@ SYMBOLS = [
     ("dm_sched_train", ctypes.c_int, [GP, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_uint64,
                                       ctypes.POINTER(ctypes.c_double)]),
     ("dm_sched_predict", ctypes.c_int, [GP, f32p, ctypes.c_size_t]),
+    ("dm_autotune", ctypes.c_int, [GP, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.POINTER(Config)]),
     ("dm_mlp_grad", ctypes.c_int, [f32p, f32p, f32p, ctypes.c_int, f32p, f32p, f32p]),
     ("dm_mlp_adam", ctypes.c_int, [f32p, f32p, f32p, ctypes.c_int, ctypes.c_int, ctypes.c_float, f32p]),
 ]
@@ -226,6 +228,12 @@ class Graph:
         _check(lib().dm_sched_train(self._h, 1 if motif else 0, int(k_train), int(epochs), int(seed),
                                     ctypes.byref(loss)))
         return loss.value
+
+    def autotune(self, motif, k, budget_ms=200.0):
+        """Tune width / LDS stack / steal granularity for (mode, k); returns the Config."""
+        c = Config()
+        _check(lib().dm_autotune(self._h, 1 if motif else 0, int(k), float(budget_ms), ctypes.byref(c)))
+        return c
 
     def sched_predict(self):
         out = np.zeros(max(self.n, 1), np.float32)

@@ -146,7 +146,7 @@ DEV WorkerLDS worker_lds(char *base, int K) {
     return w;
 }
 
-template <int W>
+template <int W, bool MOTIF>
 struct Worker {
     Grp<W> g;
     const EnumParams &p;
@@ -256,18 +256,23 @@ struct Worker {
     // level leaf_k (remaining children [lo, i]); leaf_k < 0 when not in it.
     DEV void maybe_donate(int top, int leaf_k, int &leaf_lo, int leaf_i) {
         if (!p.steal || (steps % p.steal_period) != 0) return;
+        // donate only when some worker waits, the queue cannot feed it and
+        // no static job is left to steal
         int want = 0;
         if (g.gl == 0) {
             int idle = __hip_atomic_load(&p.sc->idle, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            int left = __hip_atomic_load(&p.sc->static_left, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             uint32_t h = __hip_atomic_load(&p.sc->q_head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             uint32_t t = __hip_atomic_load(&p.sc->q_tail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             int queued = (t > h) ? (int)(t - h) : 0;
-            want = idle > queued;
+            want = left <= 0 && idle > queued;
         }
         if (!g.bcast(want, 0)) return;
+        // the last level's children are cheap: split it only in bigger pieces
+        const int leaf_min = 4 * p.steal_min;
         for (int l = job_lvl; l < top; l++) {
             int cur = ws.lv[l].cur, end = ws.lv[l].end;
-            if (end - cur >= p.steal_min) {
+            if (end - cur >= (l == K - 3 ? leaf_min : p.steal_min)) {
                 int mid = cur + ((end - cur) >> 1);
                 if (push_job(l, mid, end)) {
                     ws.lv[l].end = mid;
@@ -276,7 +281,7 @@ struct Worker {
                 return;
             }
         }
-        if (leaf_k >= 0 && leaf_i + 1 - leaf_lo >= p.steal_min) {
+        if (leaf_k >= 0 && leaf_i + 1 - leaf_lo >= leaf_min) {
             int mid = leaf_lo + ((leaf_i + 1 - leaf_lo) >> 1);
             if (push_job(leaf_k, leaf_lo, mid)) {
                 leaf_lo = mid;
@@ -693,7 +698,7 @@ struct Worker {
         lds_top = lds_lo;
         g_top = g_lo;
         cnt = 0;
-        const bool motif = p.motif != 0;
+        constexpr bool motif = MOTIF;
         if (!motif && K == 2) {
             if (g.gl == 0) {
                 int d = odeg(v0);
@@ -888,8 +893,8 @@ DEV bool steal_static(const EnumParams &p, const Grp<W> &g, int w, const WorkerL
     return false;
 }
 
-template <int W>
-__global__ __launch_bounds__(kThreads) void enum_kernel(EnumParams p) {
+template <int W, bool MOTIF>
+__global__ __launch_bounds__(kThreads, MOTIF ? 4 : 3) void enum_kernel(EnumParams p) {
     extern __shared__ __align__(16) char smem[];
     Grp<W> g;
     const int wpb = kThreads / W;
@@ -903,9 +908,9 @@ __global__ __launch_bounds__(kThreads) void enum_kernel(EnumParams p) {
     for (int i = threadIdx.x; i < p.ncls; i += kThreads) shist[i] = 0;
     __syncthreads();
 
-    unsigned long long total = 0, bytes = 0, jobs = 0;
+    unsigned long long total = 0, bytes = 0, jobs = 0, busy = 0;
     if (wid < p.nworkers) {
-        Worker<W> wk(p, ws, shist, slice, wid);
+        Worker<W, MOTIF> wk(p, ws, shist, slice, wid);
         uint32_t seed = 0x9E3779B9u * (uint32_t)(wid + 1);
         bool idle = false;
         int backoff = 1;
@@ -953,6 +958,7 @@ __global__ __launch_bounds__(kThreads) void enum_kernel(EnumParams p) {
                 const uint64_t c0 = __builtin_amdgcn_s_memtime();
                 wk.run_job(jh.lvl, jh.lo, jh.hi, !jh.donated && jh.lvl == 0 && jh.lo == 0);
                 const uint64_t c1 = __builtin_amdgcn_s_memtime();
+                busy += c1 - c0;
                 unsigned long long jc = g.lsum(wk.cnt);
                 if (g.gl == 0) {
                     total += jc;
@@ -974,6 +980,8 @@ __global__ __launch_bounds__(kThreads) void enum_kernel(EnumParams p) {
         if (total) atomicAdd(&p.sc->count, total);
         if (bytes) atomicAdd(&p.sc->alg_bytes, bytes);
         if (jobs) atomicAdd(&p.sc->jobs, jobs);
+        atomicAdd(&p.sc->busy_sum, busy);
+        atomicMax(&p.sc->busy_max, busy);
     }
 }
 
@@ -990,18 +998,18 @@ size_t enum_lds_bytes(int vwave, int threads, int K, int ncls, int lds_per_worke
     return ncls_al * 8 + (size_t)wpb * worker_lds_bytes(K) + (size_t)wpb * lds_per_worker;
 }
 
-static const void *kernel_for(int vwave) {
+static const void *kernel_for(int vwave, bool motif) {
     switch (vwave) {
-        case 8: return (const void *)enum_kernel<8>;
-        case 16: return (const void *)enum_kernel<16>;
-        case 32: return (const void *)enum_kernel<32>;
-        case 64: return (const void *)enum_kernel<64>;
+        case 8: return motif ? (const void *)enum_kernel<8, true> : (const void *)enum_kernel<8, false>;
+        case 16: return motif ? (const void *)enum_kernel<16, true> : (const void *)enum_kernel<16, false>;
+        case 32: return motif ? (const void *)enum_kernel<32, true> : (const void *)enum_kernel<32, false>;
+        case 64: return motif ? (const void *)enum_kernel<64, true> : (const void *)enum_kernel<64, false>;
         default: return nullptr;
     }
 }
 
-int enum_occupancy(int vwave, int threads, size_t lds, int *blocks_per_cu) {
-    const void *k = kernel_for(vwave);
+int enum_occupancy(int vwave, bool motif, int threads, size_t lds, int *blocks_per_cu) {
+    const void *k = kernel_for(vwave, motif);
     if (!k || threads != kThreads) return set_error(DM_ERR_ARG, "unsupported launch shape");
     if (lds > 160 * 1024) return set_error(DM_ERR_ARG, "LDS request %zu exceeds 160 KiB", lds);
     if (lds > 64 * 1024) {
@@ -1016,7 +1024,7 @@ int enum_occupancy(int vwave, int threads, size_t lds, int *blocks_per_cu) {
 }
 
 int enum_launch(const EnumParams &p, int vwave, int blocks, int threads, size_t lds, void *stream) {
-    const void *k = kernel_for(vwave);
+    const void *k = kernel_for(vwave, p.motif != 0);
     if (!k || threads != kThreads) return set_error(DM_ERR_ARG, "unsupported launch shape");
     EnumParams copy = p;
     void *args[] = {&copy};

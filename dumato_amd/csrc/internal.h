@@ -89,6 +89,8 @@ struct SchedCounters {
     uint32_t q_head, q_tail;         // donated-job queue cursors
     int32_t static_left;
     int32_t pad[3];
+    unsigned long long busy_sum;     // sum over workers of cycles spent in jobs
+    unsigned long long busy_max;     // largest per-worker busy cycles
 };
 
 struct EnumParams {
@@ -125,7 +127,7 @@ struct EnumParams {
 
 // kernels (enum_kernel.hip)
 size_t enum_lds_bytes(int vwave, int threads, int K, int ncls, int lds_per_worker);
-int enum_occupancy(int vwave, int threads, size_t lds, int *blocks_per_cu);
+int enum_occupancy(int vwave, bool motif, int threads, size_t lds, int *blocks_per_cu);
 int enum_launch(const EnumParams &p, int vwave, int blocks, int threads, size_t lds,
                 void *stream);
 int fill_u32_launch(void *ptr, uint32_t value, size_t count, void *stream);

@@ -30,6 +30,7 @@ struct CostModel {
     float y_mean = 0, y_std = 1;    // target scaling (log cycles)
     int mode_motif = 0, k_train = 0;
     double final_loss = 0;
+    float *d_params = nullptr, *d_x = nullptr, *d_out = nullptr;  // inference buffers
 };
 
 }  // namespace dm

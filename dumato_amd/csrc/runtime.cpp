@@ -166,13 +166,13 @@ int pick_shape(dm_graph *g, int K, bool motif, const dm_config &cfg, Shape &sh) 
     if (vw == 0) vw = autotuned_vwave(g, K, motif);
     if (vw != 8 && vw != 16 && vw != 32 && vw != 64)
         return set_error(DM_ERR_ARG, "vwave must be 8, 16, 32 or 64 (got %d)", vw);
-    int lds = cfg.lds_bytes > 0 ? cfg.lds_bytes : default_lds(vw);
+    int lds = cfg.lds_bytes > 0 ? cfg.lds_bytes : tuned_lds(g, K, motif, vw, default_lds(vw));
     lds = (lds + 15) & ~15;
     int ncls = 0;
     if (motif) ncls = pattern_table(K)->ncls;
     size_t lds_total = enum_lds_bytes(vw, 256, K, ncls, lds);
     int bpc = 0;
-    int r = enum_occupancy(vw, 256, lds_total, &bpc);
+    int r = enum_occupancy(vw, motif, 256, lds_total, &bpc);
     if (r) return r;
     if (bpc < 1) return set_error(DM_ERR_ARG, "launch does not fit one CU (LDS %zu B)", lds_total);
     sh.vwave = vw;
@@ -209,21 +209,36 @@ bool root_has_work(const dm_graph *g, int32_t v, int K, bool motif) {
     return g->rcsr.off[v + 1] - g->ostart[v] > 0;
 }
 
-// Plain: roots sorted by degree (descending; ties by id) dealt round-robin to
-// the workers, like the reference's v -> warp v % W partition
-// (src/main/DuMatoCPU.cu:121-135) but with the heaviest roots first.
+// Plain: roots in descending degree order (out-degree for cliques), dealt
+// round-robin to the workers like the reference's v -> warp v % W partition
+// (src/main/DuMatoCPU.cu:121-135), heaviest first.  Degree ranks are already
+// in ascending degree order, so motifs need no sort; cliques bucket by
+// out-degree.
 void plain_partition(const dm_graph *g, const std::vector<int32_t> &roots, int K, bool motif,
                      int workers, std::vector<int64_t> &sq_off, std::vector<int32_t> &sq_jobs) {
-    std::vector<int32_t> ord(roots);
-    auto wt = [&](int32_t v) -> int64_t {
-        return motif ? (g->rcsr.off[v + 1] - g->rcsr.off[v]) : (g->rcsr.off[v + 1] - g->ostart[v]);
-    };
-    std::stable_sort(ord.begin(), ord.end(), [&](int32_t a, int32_t b) { return wt(a) > wt(b); });
+    (void)K;
+    std::vector<int32_t> ord;
+    ord.reserve(roots.size());
+    if (motif) {
+        ord.assign(roots.rbegin(), roots.rend());
+    } else {
+        const int32_t mo = g->max_out;
+        std::vector<int64_t> cnt((size_t)mo + 2, 0);
+        for (int32_t v : roots) cnt[mo - (g->rcsr.off[v + 1] - g->ostart[v])]++;
+        int64_t acc = 0;
+        for (auto &c : cnt) {
+            int64_t t = c;
+            c = acc;
+            acc += t;
+        }
+        ord.resize(roots.size());
+        for (int32_t v : roots) ord[cnt[mo - (g->rcsr.off[v + 1] - g->ostart[v])]++] = v;
+    }
     sq_off.assign((size_t)workers + 1, 0);
     const int64_t n = (int64_t)ord.size();
     for (int w = 0; w < workers; w++) {
-        int64_t cnt = n > w ? (n - w + workers - 1) / workers : 0;
-        sq_off[w + 1] = sq_off[w] + cnt;
+        int64_t c = n > w ? (n - w + workers - 1) / workers : 0;
+        sq_off[w + 1] = sq_off[w] + c;
     }
     sq_jobs.assign((size_t)n * 3, 0);
     for (int w = 0; w < workers; w++) {
@@ -238,7 +253,7 @@ void plain_partition(const dm_graph *g, const std::vector<int32_t> &roots, int K
 
 // ---------------------------------------------------------------------------
 int run_enum(dm_graph *g, int K, bool motif, const dm_config &cfg, uint64_t *hist, size_t hist_len,
-             uint64_t *count_out) {
+             uint64_t *count_out, int sample_mod) {
     const double t_wall0 = now_ms();
     const PatternTable *tab = nullptr;
     if (motif) {
@@ -251,7 +266,8 @@ int run_enum(dm_graph *g, int K, bool motif, const dm_config &cfg, uint64_t *his
     } else if (K < 2 || K > kMaxK) {
         return set_error(DM_ERR_ARG, "clique size k=%d unsupported (2..%d)", K, kMaxK);
     }
-    if (cfg.steal_min < 2 || cfg.steal_period < 1) return set_error(DM_ERR_ARG, "steal_min >= 2 and steal_period >= 1 required");
+    if (cfg.steal_min == 1 || cfg.steal_min < 0 || cfg.steal_period < 1)
+        return set_error(DM_ERR_ARG, "steal_min must be 0 (tuned) or >= 2; steal_period >= 1");
     int r = ensure_device(g);
     if (r) return r;
     DeviceGraph *d = g->dev;
@@ -269,6 +285,7 @@ int run_enum(dm_graph *g, int K, bool motif, const dm_config &cfg, uint64_t *his
     for (int32_t v = 0; v < n; v++) {
         int32_t orig = g->inv[v];
         if (orig < rb || orig >= re) continue;
+        if (sample_mod > 1 && v % sample_mod != 0) continue;
         if (root_has_work(g, v, K, motif)) roots.push_back(v);
         else if (motif) skipped_bytes += 4ull * (uint64_t)(g->rcsr.off[v + 1] - g->rcsr.off[v]);
     }
@@ -344,7 +361,7 @@ int run_enum(dm_graph *g, int K, bool motif, const dm_config &cfg, uint64_t *his
     p.root_cost = want_cost ? d->d_root_cost : nullptr;
     p.root_count = want_cnt ? d->d_root_count : nullptr;
     p.steal = cfg.steal ? 1 : 0;
-    p.steal_min = cfg.steal_min;
+    p.steal_min = cfg.steal_min ? cfg.steal_min : tuned_steal_min(g, K, motif);
     p.steal_period = cfg.steal_period;
     p.bitmap = cfg.clique_bitmap ? 1 : 0;
     p.watchdog_ticks = cfg.watchdog_s > 0 ? (uint64_t)(cfg.watchdog_s * d->wall_khz * 1000.0) : 0;
@@ -403,6 +420,8 @@ int run_enum(dm_graph *g, int K, bool motif, const dm_config &cfg, uint64_t *his
     st.alg_bytes = sc1.alg_bytes + skipped_bytes;
     st.donations = sc1.donations;
     st.jobs = sc1.jobs;
+    st.busy_sum_cycles = sc1.busy_sum;
+    st.busy_max_cycles = sc1.busy_max;
     st.vwave = sh.vwave;
     st.workers = sh.workers;
     st.blocks = sh.blocks;
@@ -428,7 +447,7 @@ int dm_config_init(dm_config *cfg) {
     cfg->sched = DM_SCHED_PLAIN;
     cfg->vwave = 0;
     cfg->steal = 1;
-    cfg->steal_min = 2;
+    cfg->steal_min = 0;
     cfg->steal_period = 8;
     cfg->lds_bytes = 0;
     cfg->waves_per_block = 4;
@@ -524,7 +543,7 @@ int dm_clique_count(dm_graph *g, int k, const dm_config *cfg, uint64_t *out) {
     dm_config c;
     if (cfg) c = *cfg;
     else dm_config_init(&c);
-    return run_enum(g, k, false, c, nullptr, 0, out);
+    return run_enum(g, k, false, c, nullptr, 0, out, 1);
 }
 
 int dm_motif_count(dm_graph *g, int k, const dm_config *cfg, uint64_t *hist, size_t hist_len, uint64_t *total) {
@@ -532,7 +551,7 @@ int dm_motif_count(dm_graph *g, int k, const dm_config *cfg, uint64_t *hist, siz
     dm_config c;
     if (cfg) c = *cfg;
     else dm_config_init(&c);
-    return run_enum(g, k, true, c, hist, hist_len, total);
+    return run_enum(g, k, true, c, hist, hist_len, total, 1);
 }
 
 int dm_last_stats(const dm_graph *g, dm_stats *st) {

@@ -65,7 +65,10 @@ struct dm_graph {
     std::vector<uint64_t> last_costs, last_counts;  // original ids
     dm::CostModel *model = nullptr;
     std::vector<float> features;                    // kMlpIn per degree-rank vertex
-    int tuned_vwave[2][dm::kMaxK + 1] = {{0}};      // [motif][k], 0 = not tuned
+    // autotuned launch parameters per [motif][k]; 0 = not tuned
+    int tuned_vwave[2][dm::kMaxK + 1] = {{0}};
+    int tuned_lds[2][dm::kMaxK + 1] = {{0}};
+    int tuned_steal_min[2][dm::kMaxK + 1] = {{0}};
 };
 
 namespace dm {
@@ -76,7 +79,9 @@ bool root_has_work(const dm_graph *g, int32_t v, int K, bool motif);
 void plain_partition(const dm_graph *g, const std::vector<int32_t> &roots, int K, bool motif, int workers,
                      std::vector<int64_t> &sq_off, std::vector<int32_t> &sq_jobs);
 int run_enum(dm_graph *g, int K, bool motif, const dm_config &cfg, uint64_t *hist, size_t hist_len,
-             uint64_t *count_out);
+             uint64_t *count_out, int sample_mod = 1);
+int tuned_steal_min(const dm_graph *g, int K, bool motif);
+int tuned_lds(const dm_graph *g, int K, bool motif, int vwave, int fallback);
 
 // sched.cpp
 int autotuned_vwave(dm_graph *g, int K, bool motif);

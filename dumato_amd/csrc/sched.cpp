@@ -45,7 +45,101 @@ int autotuned_vwave(dm_graph *g, int K, bool motif) {
     return w;
 }
 
+int tuned_steal_min(const dm_graph *g, int K, bool motif) {
+    int t = g->tuned_steal_min[motif ? 1 : 0][K];
+    return t ? t : 2;
+}
+
+int tuned_lds(const dm_graph *g, int K, bool motif, int vwave, int fallback) {
+    // a tuned LDS size belongs to the tuned width
+    if (g->tuned_vwave[motif ? 1 : 0][K] == vwave && g->tuned_lds[motif ? 1 : 0][K])
+        return g->tuned_lds[motif ? 1 : 0][K];
+    return fallback;
+}
+
+// Autotuner: times candidate launch configurations on a degree-stratified
+// sample of the roots (every S-th degree rank) and keeps the fastest per
+// (mode, k).  Stage 1 picks the virtual-wave width, stage 2 the LDS stack
+// bytes per worker and the steal granularity for that width.
+int autotune(dm_graph *g, int K, bool motif, double budget_ms, dm_config *best_out) {
+    int r = ensure_device(g);
+    if (r) return r;
+    const int mi = motif ? 1 : 0;
+    std::vector<uint64_t> hist;
+    if (motif) {
+        const PatternTable *t = pattern_table(K);
+        if (!t) return last_code();
+        hist.assign(t->ncls, 0);
+    }
+    auto trial = [&](const dm_config &c, int S, double &ms) -> int {
+        uint64_t tot = 0;
+        int rr = run_enum(g, K, motif, c, motif ? hist.data() : nullptr, hist.size(), &tot, S);
+        if (rr) return rr;
+        ms = g->last.device_ms;
+        return DM_OK;
+    };
+    dm_config base;
+    dm_config_init(&base);
+    base.steal_min = 2;
+    base.vwave = 32;
+    // size the sample: aim at budget / 12 per trial
+    int S = 64;
+    double ms = 0;
+    if ((r = trial(base, S, ms))) return r;
+    const double per = std::max(budget_ms / 12.0, 0.5);
+    while (ms > per && S < (1 << 20)) {
+        S *= 4;
+        if ((r = trial(base, S, ms))) return r;
+    }
+    while (ms * 4 < per && S > 1) {
+        S = std::max(1, S / 4);
+        if ((r = trial(base, S, ms))) return r;
+    }
+    double best = 1e300;
+    dm_config bc = base;
+    for (int w : {8, 16, 32, 64}) {
+        dm_config c = base;
+        c.vwave = w;
+        double t1 = 0, t2 = 0;
+        if ((r = trial(c, S, t1)) || (r = trial(c, S, t2))) return r;
+        double t = std::min(t1, t2);
+        if (t < best) {
+            best = t;
+            bc = c;
+        }
+    }
+    const int dl = 32768 / (256 / bc.vwave);
+    for (int lds : {dl / 2, dl, dl * 2}) {
+        for (int sm : {2, 8}) {
+            dm_config c = bc;
+            c.lds_bytes = lds;
+            c.steal_min = sm;
+            double t1 = 0, t2 = 0;
+            if ((r = trial(c, S, t1)) || (r = trial(c, S, t2))) continue;  // e.g. LDS too large
+            double t = std::min(t1, t2);
+            if (t < best) {
+                best = t;
+                bc = c;
+            }
+        }
+    }
+    if (bc.lds_bytes == 0) bc.lds_bytes = dl;
+    g->tuned_vwave[mi][K] = bc.vwave;
+    g->tuned_lds[mi][K] = bc.lds_bytes;
+    g->tuned_steal_min[mi][K] = bc.steal_min;
+    if (best_out) {
+        *best_out = bc;
+        best_out->vwave = bc.vwave;
+    }
+    return DM_OK;
+}
+
 void free_model(dm_graph *g) {
+    if (g->model) {
+        if (g->model->d_params) (void)hipFree(g->model->d_params);
+        if (g->model->d_x) (void)hipFree(g->model->d_x);
+        if (g->model->d_out) (void)hipFree(g->model->d_out);
+    }
     delete g->model;
     g->model = nullptr;
 }
@@ -240,33 +334,28 @@ static int train_model(const std::vector<float> &X, const std::vector<float> &Y,
     return DM_OK;
 }
 
-// Predicted log-cycles per degree-rank vertex.
+// Predicted log-cycles per degree-rank vertex.  The standardised features
+// and the parameters stay on the device after the first call; each call is
+// one forward launch and one copy back.
 static int predict_all(dm_graph *g, std::vector<float> &pred) {
-    const CostModel &M = *g->model;
+    CostModel &M = *g->model;
     const int32_t n = g->rcsr.n;
-    std::vector<float> X((size_t)n * kMlpIn);
-    for (int32_t v = 0; v < n; v++)
-        for (int i = 0; i < kMlpIn; i++)
-            X[(size_t)v * kMlpIn + i] = (g->features[(size_t)v * kMlpIn + i] - M.feat_mean[i]) / M.feat_std[i];
-    float *dP = nullptr, *dX = nullptr, *dO = nullptr;
-    auto cleanup = [&]() {
-        if (dP) (void)hipFree(dP);
-        if (dX) (void)hipFree(dX);
-        if (dO) (void)hipFree(dO);
-    };
-    if (hipMalloc(&dP, kMlpParams * 4) != hipSuccess || hipMalloc(&dX, X.size() * 4 + 4) != hipSuccess ||
-        hipMalloc(&dO, (size_t)n * 4 + 4) != hipSuccess) {
-        cleanup();
-        return set_error(DM_ERR_NOMEM, "MLP inference buffers");
+    if (!M.d_params) {
+        std::vector<float> X((size_t)n * kMlpIn);
+        for (int32_t v = 0; v < n; v++)
+            for (int i = 0; i < kMlpIn; i++)
+                X[(size_t)v * kMlpIn + i] = (g->features[(size_t)v * kMlpIn + i] - M.feat_mean[i]) / M.feat_std[i];
+        if (hipMalloc(&M.d_params, kMlpParams * 4) != hipSuccess ||
+            hipMalloc(&M.d_x, X.size() * 4 + 4) != hipSuccess || hipMalloc(&M.d_out, (size_t)n * 4 + 4) != hipSuccess)
+            return set_error(DM_ERR_NOMEM, "MLP inference buffers");
+        HIPCHK(hipMemcpy(M.d_params, M.params.data(), kMlpParams * 4, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(M.d_x, X.data(), X.size() * 4, hipMemcpyHostToDevice));
     }
-    hipMemcpy(dP, M.params.data(), kMlpParams * 4, hipMemcpyHostToDevice);
-    hipMemcpy(dX, X.data(), X.size() * 4, hipMemcpyHostToDevice);
-    int r = mlp_forward_launch(dP, dX, n, dO, nullptr);
-    pred.assign(n, 0.f);
-    if (!r && hipMemcpy(pred.data(), dO, (size_t)n * 4, hipMemcpyDeviceToHost) != hipSuccess)
-        r = set_error(DM_ERR_HIP, "MLP inference copy");
-    cleanup();
+    int r = mlp_forward_launch(M.d_params, M.d_x, n, M.d_out, g->dev->stream);
     if (r) return r;
+    pred.resize(n);
+    HIPCHK(hipMemcpyAsync(pred.data(), M.d_out, (size_t)n * 4, hipMemcpyDeviceToHost, g->dev->stream));
+    HIPCHK(hipStreamSynchronize(g->dev->stream));
     for (auto &p : pred) p = p * M.y_std + M.y_mean;
     return DM_OK;
 }
@@ -324,17 +413,48 @@ int learned_partition(dm_graph *g, const std::vector<int32_t> &roots, int K, boo
             lo = hi;
         }
     }
-    std::sort(pieces.begin(), pieces.end(), [](const Piece &a, const Piece &b) { return a.cost > b.cost; });
+    // descending cost order by bucketing log-cost (O(n)); exact order within
+    // a bucket does not matter for the binning below
+    const int B = 4096;
+    double lmax = -1e300, lmin = 1e300;
+    for (const Piece &p : pieces) {
+        double l = std::log(std::max(p.cost, 1e-300));
+        lmax = std::max(lmax, l);
+        lmin = std::min(lmin, l);
+    }
+    const double scale = (lmax > lmin) ? (B - 1) / (lmax - lmin) : 0.0;
+    std::vector<int32_t> bucket_of(pieces.size()), start(B + 1, 0), order(pieces.size());
+    for (size_t i = 0; i < pieces.size(); i++) {
+        double l = std::log(std::max(pieces[i].cost, 1e-300));
+        int bkt = (int)((lmax - l) * scale);
+        bucket_of[i] = std::min(std::max(bkt, 0), B - 1);
+        start[bucket_of[i] + 1]++;
+    }
+    for (int b2 = 0; b2 < B; b2++) start[b2 + 1] += start[b2];
+    for (size_t i = 0; i < pieces.size(); i++) order[start[bucket_of[i]]++] = (int32_t)i;
+    // LPT (least-loaded worker first) for the heaviest pieces, then a snake
+    // deal of the light tail
     using QE = std::pair<double, int>;
     std::priority_queue<QE, std::vector<QE>, std::greater<QE>> heap;
     for (int w = 0; w < workers; w++) heap.push({0.0, w});
     std::vector<std::vector<int32_t>> q(workers);
-    for (const Piece &p : pieces) {
-        QE t = heap.top();
+    const size_t head = std::min(order.size(), (size_t)workers * 2);
+    for (size_t t = 0; t < head; t++) {
+        QE e = heap.top();
         heap.pop();
-        q[t.second].push_back((int32_t)(&p - pieces.data()));
-        t.first += p.cost;
-        heap.push(t);
+        q[e.second].push_back(order[t]);
+        e.first += pieces[order[t]].cost;
+        heap.push(e);
+    }
+    std::vector<int> by_load(workers);
+    for (int w = 0; w < workers; w++) {
+        by_load[w] = heap.top().second;  // least loaded first
+        heap.pop();
+    }
+    for (size_t t = head; t < order.size(); t++) {
+        size_t i = t - head, lap = i / workers, pos = i % workers;
+        int w = by_load[(lap & 1) ? workers - 1 - pos : pos];
+        q[w].push_back(order[t]);
     }
     sq_off.assign((size_t)workers + 1, 0);
     for (int w = 0; w < workers; w++) sq_off[w + 1] = sq_off[w] + (int64_t)q[w].size();
@@ -425,6 +545,12 @@ using namespace dm;
 extern "C" int dm_sched_train(dm_graph *g, int mode_motif, int k_train, int epochs, uint64_t seed,
                               double *final_loss) {
     return sched_train(g, mode_motif, k_train, epochs, seed, final_loss);
+}
+
+extern "C" int dm_autotune(dm_graph *g, int mode_motif, int k, double budget_ms, dm_config *best) {
+    if (!g) return set_error(DM_ERR_ARG, "graph is NULL");
+    if (budget_ms <= 0) return set_error(DM_ERR_ARG, "budget_ms must be > 0");
+    return autotune(g, k, mode_motif != 0, budget_ms, best);
 }
 
 extern "C" int dm_sched_predict(dm_graph *g, float *pred, size_t n) {

@@ -42,7 +42,8 @@ typedef struct dm_config {
     int32_t sched;           /* DM_SCHED_PLAIN or DM_SCHED_LEARNED */
     int32_t vwave;           /* lanes per DFS worker: 8, 16, 32, 64; 0 = autotuned */
     int32_t steal;           /* 1: idle workers receive donated subtrees */
-    int32_t steal_min;       /* donate only levels with >= steal_min children left */
+    int32_t steal_min;       /* donate only levels with >= steal_min children left
+                                (the last level: >= 4 * steal_min); 0 = autotuned */
     int32_t steal_period;    /* DFS steps between checks for idle workers */
     int32_t lds_bytes;       /* LDS stack bytes per worker; 0 = autotuned */
     int32_t waves_per_block; /* 0 = 4 */
@@ -65,6 +66,8 @@ typedef struct dm_stats {
     uint64_t alg_bytes;       /* algorithmic bytes of the enumeration */
     uint64_t donations;       /* subtrees donated to idle workers */
     uint64_t jobs;            /* jobs executed (static + donated) */
+    uint64_t busy_sum_cycles; /* sum over workers of cycles spent inside jobs */
+    uint64_t busy_max_cycles; /* largest per-worker busy cycles (makespan proxy) */
     int32_t vwave, workers, blocks, lds_bytes;
 } dm_stats;
 
@@ -132,6 +135,14 @@ int dm_pattern_table(int k, uint16_t *quick_to_cls, size_t nquick,
  * is kept in g and used by later calls with sched = DM_SCHED_LEARNED. */
 int dm_sched_train(dm_graph *g, int mode_motif, int k_train, int epochs,
                    uint64_t seed, double *final_loss);
+/* Autotuner (no reference counterpart; the reference fixes the virtual-warp
+ * width per binary and takes donationsPerWarp/threshold from argv): times
+ * candidate widths (8/16/32/64), LDS stack sizes and steal granularities on
+ * a degree-stratified sample of the roots within about budget_ms of device
+ * time, and keeps the fastest for (mode, k) in g.  Later calls with vwave,
+ * lds_bytes or steal_min = 0 use it.  *best (nullable) receives it. */
+int dm_autotune(dm_graph *g, int mode_motif, int k, double budget_ms,
+                dm_config *best);
 /* Predicted cost (model output, log-cycles) per original vertex id. */
 int dm_sched_predict(dm_graph *g, float *pred, size_t n);
 /* The MLP kernels on their own, for checking against a CPU reference.

@@ -68,3 +68,17 @@ def test_learned_without_model_is_an_error():
     g = dm.Graph.erdos_renyi(200, 800, 1)
     with pytest.raises(dm.DMError):
         g.clique_count(3, dm.config(sched=dm.DM_SCHED_LEARNED))
+
+
+def test_autotune_keeps_counts(graph):
+    g, ref = graph
+    print("case autotune", flush=True)
+    best = g.autotune(False, 5, budget_ms=50.0)
+    assert best.vwave in (8, 16, 32, 64) and best.lds_bytes > 0 and best.steal_min >= 2
+    assert g.clique_count(5) == O.clique_count(ref, 5)
+    st = g.last_stats()
+    assert st["vwave"] == best.vwave and st["lds_bytes"] == best.lds_bytes
+    bm = g.autotune(True, 4, budget_ms=50.0)
+    tot, hist = g.motif_count(4)
+    wt, wh = O.motif_count(ref, 4)
+    assert tot == wt and np.array_equal(hist, wh) and g.last_stats()["vwave"] == bm.vwave

@@ -32,7 +32,7 @@ def test_exports_every_declared_symbol():
 
 def test_config_defaults():
     c = dm.config()
-    assert c.sched == dm.DM_SCHED_PLAIN and c.steal == 1 and c.steal_min >= 2
+    assert c.sched == dm.DM_SCHED_PLAIN and c.steal == 1 and c.steal_min == 0
     assert c.vwave == 0 and c.watchdog_s > 0
 
 
