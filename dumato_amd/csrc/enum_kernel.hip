@@ -227,11 +227,18 @@ struct Worker {
         uint32_t slot = 0;
         if (g.gl == 0) {
             uint32_t t = __hip_atomic_load(&p.sc->q_tail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (t < p.q_cap && atomicCAS(&p.sc->q_tail, t, t + 1) == t) {
-                ok = 1;
-                slot = t;
+            if (t < p.q_cap) {
+                // count the job before it becomes visible, so outstanding
+                // cannot reach zero while it waits in the queue
                 atomicAdd(&p.sc->outstanding, 1ull);
-                atomicAdd(&p.sc->donations, 1ull);
+                t = atomicAdd(&p.sc->q_tail, 1u);
+                if (t < p.q_cap) {
+                    ok = 1;
+                    slot = t;
+                    atomicAdd(&p.sc->donations, 1ull);
+                } else {
+                    atomicSub(&p.sc->outstanding, 1ull);
+                }
             }
         }
         ok = g.bcast(ok, 0);
@@ -264,7 +271,7 @@ struct Worker {
             int left = __hip_atomic_load(&p.sc->static_left, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             uint32_t h = __hip_atomic_load(&p.sc->q_head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             uint32_t t = __hip_atomic_load(&p.sc->q_tail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            int queued = (t > h) ? (int)(t - h) : 0;
+            int queued = (t > h) ? (int)(t - h) : 0;  // h > t: tickets waiting for jobs
             want = left <= 0 && idle > queued;
         }
         if (!g.bcast(want, 0)) return;
@@ -554,63 +561,68 @@ struct Worker {
     // every node above the last level, so the algorithmic bytes (sum of
     // out-degrees of the children of nodes with 1..k-2 vertices) are summed
     // in the same loops from dp[] = out-degrees.
+    // Work of one second-level pair (i, j), j in row i: the (k-2)-level
+    // candidate set S2 = row_i & row_j, then the deeper levels by AND/popcount.
     template <int NW>
-    DEV void bitmap_lane(const uint64_t *rows, const int32_t *dp, int i, unsigned long long &c,
+    DEV void bitmap_pair(const uint64_t *rows, const int32_t *dp, int i, int j, unsigned long long &c,
                          unsigned long long &by) const {
-        uint64_t S1[NW];
+        uint64_t S2[NW];
 #pragma unroll
-        for (int w = 0; w < NW; w++) S1[w] = rows[i * NW + w];
+        for (int w = 0; w < NW; w++) S2[w] = rows[i * NW + w] & rows[j * NW + w];
+        if (K == 4) {
+            int pc = 0;
 #pragma unroll
-        for (int w1 = 0; w1 < NW; w1++) {
-            uint64_t m1 = S1[w1];
-            while (m1) {
-                const int j = w1 * 64 + __builtin_ctzll(m1);
-                m1 &= m1 - 1;
-                by += (unsigned long long)dp[j];
-                if (K == 4) {
+            for (int w = 0; w < NW; w++) pc += __popcll(S2[w]);
+            c += (unsigned long long)pc;
+            return;
+        }
+#pragma unroll
+        for (int w2 = 0; w2 < NW; w2++) {
+            uint64_t m2 = S2[w2];
+            while (m2) {
+                const int k = w2 * 64 + __builtin_ctzll(m2);
+                m2 &= m2 - 1;
+                by += (unsigned long long)dp[k];
+                if (K == 5) {
                     int pc = 0;
 #pragma unroll
-                    for (int w = 0; w < NW; w++) pc += __popcll(S1[w] & rows[j * NW + w]);
+                    for (int w = 0; w < NW; w++) pc += __popcll(S2[w] & rows[k * NW + w]);
                     c += (unsigned long long)pc;
                     continue;
                 }
-                uint64_t S2[NW];
+                uint64_t S3[NW];
 #pragma unroll
-                for (int w = 0; w < NW; w++) S2[w] = S1[w] & rows[j * NW + w];
+                for (int w = 0; w < NW; w++) S3[w] = S2[w] & rows[k * NW + w];
+                int pc = 0;
 #pragma unroll
-                for (int w2 = 0; w2 < NW; w2++) {
-                    uint64_t m2 = S2[w2];
-                    while (m2) {
-                        const int k = w2 * 64 + __builtin_ctzll(m2);
-                        m2 &= m2 - 1;
-                        by += (unsigned long long)dp[k];
-                        if (K == 5) {
-                            int pc = 0;
+                for (int w3 = 0; w3 < NW; w3++) {
+                    uint64_t m3 = S3[w3];
+                    while (m3) {
+                        const int q = w3 * 64 + __builtin_ctzll(m3);
+                        m3 &= m3 - 1;
+                        by += (unsigned long long)dp[q];
 #pragma unroll
-                            for (int w = 0; w < NW; w++) pc += __popcll(S2[w] & rows[k * NW + w]);
-                            c += (unsigned long long)pc;
-                            continue;
-                        }
-                        uint64_t S3[NW];
-#pragma unroll
-                        for (int w = 0; w < NW; w++) S3[w] = S2[w] & rows[k * NW + w];
-                        int pc = 0;
-#pragma unroll
-                        for (int w3 = 0; w3 < NW; w3++) {
-                            uint64_t m3 = S3[w3];
-                            while (m3) {
-                                const int q = w3 * 64 + __builtin_ctzll(m3);
-                                m3 &= m3 - 1;
-                                by += (unsigned long long)dp[q];
-#pragma unroll
-                                for (int w = 0; w < NW; w++) pc += __popcll(S3[w] & rows[q * NW + w]);
-                            }
-                        }
-                        c += (unsigned long long)pc;
+                        for (int w = 0; w < NW; w++) pc += __popcll(S3[w] & rows[q * NW + w]);
                     }
                 }
+                c += (unsigned long long)pc;
             }
         }
+    }
+
+    // position of the r-th set bit (0-based) of m; m has more than r bits
+    DEV static int select64(uint64_t m, int r) {
+        int pos = 0;
+#pragma unroll
+        for (int sh = 32; sh >= 1; sh >>= 1) {
+            const int c = __popcll(m & ((1ull << sh) - 1));
+            if (r >= c) {
+                r -= c;
+                m >>= sh;
+                pos += sh;
+            }
+        }
+        return pos;
     }
 
     template <int NW>
@@ -665,7 +677,8 @@ struct Worker {
         }
         wsync();
         if (first_piece && g.gl == 0) bytes += 4ull * (unsigned long long)d;
-        // first-level children [lo, hi): one per lane, in batches of W
+        // first-level children [lo, hi) in batches of W; each batch is
+        // flattened into its (i, j) pairs, which the lanes stride over
         R.cur = lo;
         R.end = hi;
         unsigned long long by = 0;
@@ -677,11 +690,37 @@ struct Worker {
             if (cur >= end) break;
             const int take = end - cur < W ? end - cur : W;
             R.cur = cur + take;
+            int np = 0;
             if (g.gl < take) {
                 const int i = cur + g.gl;
                 by += (unsigned long long)dp[i];
-                bitmap_lane<NW>(rows, dp, i, cnt, by);
+#pragma unroll
+                for (int w = 0; w < NW; w++) np += __popcll(rows[i * NW + w]);
             }
+            const int sc = g.iscan(np);
+            if (g.gl < take) pre[g.gl + 1] = sc;
+            if (g.gl == 0) pre[0] = 0;
+            const int P = g.bcast(sc, W - 1);
+            wsync();
+            int a = 0;
+            for (int t = g.gl; t < P; t += W) {
+                while (pre[a + 1] <= t) a++;
+                const int i = cur + a;
+                int r = t - pre[a];
+                int w = 0;
+                uint64_t m = rows[i * NW];
+                while (true) {
+                    const int c = __popcll(m);
+                    if (r < c || w == NW - 1) break;
+                    r -= c;
+                    w++;
+                    m = rows[i * NW + w];
+                }
+                const int j = w * 64 + select64(m, r);
+                by += (unsigned long long)dp[j];
+                bitmap_pair<NW>(rows, dp, i, j, cnt, by);
+            }
+            wsync();
         }
         bytes += 4ull * by;
         lds_top = R.mark_lds;
@@ -829,61 +868,59 @@ DEV bool take_static(const EnumParams &p, const Grp<W> &g, int w, const WorkerLD
     return true;
 }
 
+// Donated jobs are served in ticket order: an idle worker takes one ticket
+// (atomicAdd on q_head) and from then on polls only the slot of its ticket,
+// so waiting workers never contend on a shared word.  A slot is complete when
+// all its words carry this launch's epoch.  Returns true when the ticket's job
+// has arrived (and clears the ticket).
 template <int W>
-DEV bool pop_donated(const EnumParams &p, const Grp<W> &g, const WorkerLDS &ws, JobHead &jh, uint64_t t0) {
-    int got = 0;
-    uint32_t slot = 0;
-    if (g.gl == 0) {
-        uint32_t h = __hip_atomic_load(&p.sc->q_head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        uint32_t t = __hip_atomic_load(&p.sc->q_tail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (h < t && h < p.q_cap && atomicCAS(&p.sc->q_head, h, h + 1) == h) {
-            got = 1;
-            slot = h;
-        }
+DEV bool poll_ticket(const EnumParams &p, const Grp<W> &g, const WorkerLDS &ws, JobHead &jh, bool &has_ticket,
+                     uint32_t &ticket) {
+    if (!has_ticket) {
+        uint32_t t = 0;
+        if (g.gl == 0) t = atomicAdd(&p.sc->q_head, 1u);
+        ticket = (uint32_t)g.bcast((int)t, 0);
+        has_ticket = true;
     }
-    got = g.bcast(got, 0);
-    if (!got) return false;
-    slot = (uint32_t)g.bcast((int)slot, 0);
-    // the pusher reserved the slot before writing it: wait until every word
-    // carries this launch's epoch
-    const uint64_t *src = p.q[slot].w;
+    if (ticket >= p.q_cap) return false;  // no slot: wait for the end
+    const uint64_t *src = p.q[ticket].w;
     const uint32_t ep = (uint32_t)p.epoch;
-    jh.lvl = jh.lo = jh.hi = 0;
-    while (true) {
-        bool mine_ok = true;
-        for (int j = g.gl; j < kJobWords; j += W) {
-            uint64_t v = __hip_atomic_load(src + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if ((uint32_t)(v >> 32) != ep) {
-                mine_ok = false;
-                continue;
-            }
-            int32_t x = (int32_t)(uint32_t)v;
-            if (j >= 4) ws.jid[j - 4] = x;
-            else if (j == 0) jh.lvl = x;
-            else if (j == 1) jh.lo = x;
-            else if (j == 2) jh.hi = x;
+    int v[(kJobWords + W - 1) / W];
+    bool mine_ok = true;
+#pragma unroll
+    for (int r = 0; r < (kJobWords + W - 1) / W; r++) {
+        const int j = g.gl + r * W;
+        v[r] = 0;
+        if (j < kJobWords) {
+            uint64_t x = __hip_atomic_load(src + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if ((uint32_t)(x >> 32) != ep) mine_ok = false;
+            v[r] = (int32_t)(uint32_t)x;
         }
-        if (g.ballot(!mine_ok) == 0) break;
-        if (p.watchdog_ticks && now_ticks() - t0 > p.watchdog_ticks) {
-            if (g.gl == 0) atomicCAS(&p.sc->error, 0, 1);
-            return false;
-        }
-        __builtin_amdgcn_s_sleep(1);
     }
-    jh.lvl = g.bcast(jh.lvl, 0);
-    jh.lo = g.bcast(jh.lo, 1);
-    jh.hi = g.bcast(jh.hi, 2);
+    if (g.ballot(!mine_ok) != 0) return false;
+#pragma unroll
+    for (int r = 0; r < (kJobWords + W - 1) / W; r++) {
+        const int j = g.gl + r * W;
+        if (j >= 4 && j < kJobWords) ws.jid[j - 4] = v[r];
+    }
+    jh.lvl = g.bcast(v[0], 0);
+    jh.lo = g.bcast(v[0], 1);
+    jh.hi = g.bcast(v[0], 2);
     jh.donated = true;
+    has_ticket = false;
     wsync();
     return true;
 }
 
 template <int W>
 DEV bool steal_static(const EnumParams &p, const Grp<W> &g, int w, const WorkerLDS &ws, JobHead &jh,
-                      uint32_t &seed) {
+                      uint32_t &seed, bool &static_done) {
     int left = 0;
     if (g.gl == 0) left = __hip_atomic_load(&p.sc->static_left, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (g.bcast(left, 0) <= 0) return false;
+    if (g.bcast(left, 0) <= 0) {
+        static_done = true;
+        return false;
+    }
     for (int probe = 0; probe < 8; probe++) {
         seed = seed * 1664525u + 1013904223u;
         int v = (int)((seed >> 8) % (uint32_t)p.nworkers);
@@ -912,14 +949,18 @@ __global__ __launch_bounds__(kThreads, MOTIF ? 4 : 3) void enum_kernel(EnumParam
     if (wid < p.nworkers) {
         Worker<W, MOTIF> wk(p, ws, shist, slice, wid);
         uint32_t seed = 0x9E3779B9u * (uint32_t)(wid + 1);
-        bool idle = false;
+        bool idle = false, static_done = false, has_ticket = false;
+        uint32_t ticket = 0;
         int backoff = 1;
         const uint64_t t0 = wk.t_start;
         while (true) {
             JobHead jh;
-            bool got = take_static<W>(p, g, wid, ws, jh);
-            if (!got) got = pop_donated<W>(p, g, ws, jh, t0);
-            if (!got) got = steal_static<W>(p, g, wid, ws, jh, seed);
+            bool got = false;
+            if (!static_done) {
+                got = take_static<W>(p, g, wid, ws, jh);
+                if (!got) got = steal_static<W>(p, g, wid, ws, jh, seed, static_done);
+            }
+            if (!got && static_done) got = poll_ticket<W>(p, g, ws, jh, has_ticket, ticket);
             if (!got) {
                 if (!idle) {
                     if (g.gl == 0) atomicAdd(&p.sc->idle, 1);

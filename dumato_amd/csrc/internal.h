@@ -78,17 +78,26 @@ struct Job {
     uint64_t w[kJobWords];
 };
 
+// Scheduler counters.  Fields that idle and busy workers poll or update
+// concurrently each sit on their own 128-byte line, so traffic on one never
+// stalls the others.
 struct SchedCounters {
     unsigned long long outstanding;  // unfinished jobs (static + donated)
+    char pad0[120];
+    int32_t idle;                    // workers waiting for work
+    char pad1[124];
+    int32_t static_left;             // static jobs not yet claimed
+    char pad2[124];
+    uint32_t q_head;                 // next ticket of the donated-job queue
+    char pad3[124];
+    uint32_t q_tail;                 // next free slot of the donated-job queue
+    char pad4[124];
+    int32_t error;                   // 1 watchdog, 2 arena overflow, 3 bad job
+    char pad5[124];
     unsigned long long donations;
     unsigned long long jobs;
     unsigned long long alg_bytes;
     unsigned long long count;        // clique count / motif subgraph total
-    int32_t idle;                    // workers waiting for work
-    int32_t error;                   // 1 watchdog, 2 arena overflow, 3 bad job
-    uint32_t q_head, q_tail;         // donated-job queue cursors
-    int32_t static_left;
-    int32_t pad[3];
     unsigned long long busy_sum;     // sum over workers of cycles spent in jobs
     unsigned long long busy_max;     // largest per-worker busy cycles
 };
