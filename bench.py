@@ -1,20 +1,28 @@
 #!/usr/bin/env python3
 """Benchmark: subgraphs enumerated per second on one MI355X.
 
-Headline workload (value): k=4 motif histogram on the seeded Chung-Lu graph
-(n=18,000, 200,000 edges, gamma 2.3, wmax 500).  Also reported in the same
-JSON line: k=5 clique count on the seeded R-MAT scale-17 graph, and the
-learned-scheduler speedup where a model is trained.
+Headline (value): k=4 motif histogram of the seeded Chung-Lu graph
+(n=18,000, 200,000 edges, gamma 2.3, wmax 500), plain scheduler.
 
-A step is one full counting call (schedule + kernel + result copy) over the
-whole graph, which is resident on the GPU before timing starts.  value =
-subgraphs counted per second of wall time over the timed steps.  The roofline
-uses the enumeration kernel's own duration (HIP events on the library's
-stream) and its algorithmic bytes (counted on the device with the formula of
-the measurement plan; checked against the CPU oracle in tests).
+Same JSON line, extra fields:
+  clique_rmat17   k=5 and k=6 clique counts of the seeded R-MAT scale-17
+                  graph, plain vs learned scheduler (the cost model is trained
+                  on per-root cycle counts of a k=4 run of the same graph);
+  learned_speedup plain / learned time per query, where time = enumeration
+                  kernel + host scheduling (inference, ordering, binning);
+  motif5_ba       (--full) k=5 motif histogram of the Barabasi-Albert graph
+                  with 200 planted cliques, plain vs learned.
+
+A step is one full counting call over the whole graph (schedule, kernel,
+result copy); the graph is resident on the GPU before timing starts and the
+launch parameters are autotuned once before timing.  value = subgraphs per
+second of wall time over the timed steps.  The roofline uses the enumeration
+kernel's own duration (HIP events on the library's stream, median over the
+steps) and its algorithmic bytes (the measurement plan's formula, summed on
+the device and checked against the CPU oracle in tests).
 
 The CPU baseline is the oracle's port of the reference's DFS logic
-(clique_DM_DFS / motifs_DM_DFS: thread per root, linear-scan adjacency),
+(clique_DM_DFS / motifs_DM_DFS: thread per root, linear-scan adjacency), with
 OpenMP over all host cores, timed on a seeded random sample of roots for a
 bounded time.
 """
@@ -65,29 +73,61 @@ def cpu_baseline(graph, mode, k, budget_s, seed=7):
                       "faithful linear-scan DFS of the reference (%s k=%d)" % (done, n, count, dt, mode, k)}
 
 
-def run_steps(fn, warmup, steps):
+def timed(fn, graph, warmup, steps):
+    """Run fn warmup+steps times; returns (last result, wall s, kernel ms list, sched ms list)."""
     for _ in range(warmup):
         fn()
-    dev, walls, out = [], [], None
+    kms, sms = [], []
+    out = None
     t0 = time.perf_counter()
     for _ in range(steps):
-        s0 = time.perf_counter()
         out = fn()
-        walls.append(time.perf_counter() - s0)
-        dev.append(out[1])
-    t = time.perf_counter() - t0
-    return out, t, dev, walls
+        st = graph.last_stats()
+        kms.append(st["device_ms"])
+        sms.append(st["sched_ms"])
+    return out, time.perf_counter() - t0, kms, sms
+
+
+def query_block(graph, mode, k, sched, warmup, steps):
+    import dumato_amd as dm
+    cfg = dm.config(sched=sched)
+    fn = (lambda: graph.motif_count(k, cfg)[0]) if mode == "motif" else (lambda: graph.clique_count(k, cfg))
+    count, wall, kms, sms = timed(fn, graph, warmup, steps)
+    st = graph.last_stats()
+    km, sm = statistics.median(kms), statistics.median(sms)
+    return {"subgraphs": count, "kernel_ms": km, "sched_ms": sm, "query_ms": km + sm,
+            "subgraphs_per_s": count / ((km + sm) / 1e3), "wall_ms_per_step": 1e3 * wall / steps,
+            "vwave": st["vwave"], "workers": st["workers"], "donations": st["donations"],
+            "balance": st["busy_sum_cycles"] / max(1, st["workers"] * st["busy_max_cycles"]),
+            "alg_GBs": st["alg_bytes"] / (km / 1e3) / 1e9}
+
+
+def plain_vs_learned(graph, mode, k_list, k_train, warmup, steps, budget_ms=300.0):
+    import dumato_amd as dm
+    out = {}
+    t = time.perf_counter()
+    loss = graph.sched_train(mode == "motif", k_train, epochs=30, seed=1)
+    out["train"] = {"k_train": k_train, "loss": loss, "seconds": time.perf_counter() - t}
+    for k in k_list:
+        graph.autotune(mode == "motif", k, budget_ms)
+        p = query_block(graph, mode, k, dm.DM_SCHED_PLAIN, warmup, steps)
+        lq = query_block(graph, mode, k, dm.DM_SCHED_LEARNED, warmup, steps)
+        assert p["subgraphs"] == lq["subgraphs"], "learned and plain counts differ"
+        out["k%d" % k] = {"plain": p, "learned": lq,
+                          "learned_speedup": p["query_ms"] / lq["query_ms"],
+                          "learned_kernel_speedup": p["kernel_ms"] / lq["kernel_ms"]}
+    return out
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=7)
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
-    ap.add_argument("--vwave", type=int, default=0)
+    ap.add_argument("--full", action="store_true", help="also the k=5 motif run on the BA graph (~1 min)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -100,23 +140,22 @@ def main():
 
     g = W.chunglu_18k()
     K = 4
-    cfg = dm.config(vwave=args.vwave)
+    g.autotune(True, K, 300.0)
+    cfg = dm.config()
 
     def step():
-        tot, hist = g.motif_count(K, cfg)
-        st = g.last_stats()
-        return tot, st["device_ms"], st
+        return g.motif_count(K, cfg)[0]
 
-    (total, _, st), elapsed, dev, walls = run_steps(step, args.warmup, args.steps)
+    total, elapsed, kms, _ = timed(step, g, args.warmup, args.steps)
+    st = g.last_stats()
     value = args.steps * total / elapsed
-    kern_s = statistics.mean(dev) / 1e3
-    achieved = st["alg_bytes"] / kern_s / 1e9
+    kern_ms = statistics.median(kms)
+    achieved = st["alg_bytes"] / (kern_ms / 1e3) / 1e9
     traffic = None
     if os.path.exists(PROFILE_TRAFFIC):
-        tr = json.load(open(PROFILE_TRAFFIC))
-        traffic = tr.get("motif4_chunglu18k")
+        traffic = json.load(open(PROFILE_TRAFFIC)).get("motif4_chunglu18k")
     line = {
-        "metric": "subgraphs enumerated/sec (k=4 motif on Chung-Lu 18K)",
+        "metric": "subgraphs enumerated/sec (k=4 motif on Chung-Lu 18K; k=5 clique and learned speedup below)",
         "value": value,
         "unit": "subgraphs/s",
         "n_gpus": 1,
@@ -128,29 +167,27 @@ def main():
         "vs_baseline": None,
         "dtype": "int64",
         "data": "synthetic (seeded Chung-Lu n=18000 m=200000 gamma=2.3 wmax=500)",
-        "config": {"workload": "motif k=4, Chung-Lu n=18000 m=200000 gamma=2.3 wmax=500, seed %d" % (W.BASE_SEED + 2),
-                   "subgraphs": total, "vwave": st["vwave"], "workers": st["workers"],
-                   "kernel_ms": statistics.mean(dev), "kernel_ms_median": statistics.median(dev),
-                   "subgraphs_per_kernel_s": total / kern_s, "scheduler": "plain"},
+        "config": {"workload": "motif k=4, Chung-Lu n=18000 m=200000 gamma=2.3 wmax=500, seed %d"
+                               % (W.BASE_SEED + 2),
+                   "subgraphs": total, "scheduler": "plain", "vwave": st["vwave"],
+                   "workers": st["workers"], "lds_bytes": st["lds_bytes"],
+                   "kernel_ms_median": kern_ms, "kernel_ms_min": min(kms),
+                   "subgraphs_per_kernel_s": total / (kern_ms / 1e3),
+                   "balance": st["busy_sum_cycles"] / max(1, st["workers"] * st["busy_max_cycles"]),
+                   "donations": st["donations"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "alg_bytes": st["alg_bytes"]},
     }
     if not args.no_extra:
         gr = W.rmat_s17()
-        ccfg = dm.config(vwave=args.vwave)
-
-        def cstep():
-            c = gr.clique_count(5, ccfg)
-            s = gr.last_stats()
-            return c, s["device_ms"], s
-
-        (c5, _, cst), cel, cdev, _ = run_steps(cstep, 1, max(2, args.steps // 2))
-        line["clique5_rmat17"] = {"subgraphs": c5, "value": len(cdev) * c5 / cel, "unit": "subgraphs/s",
-                                  "kernel_ms": statistics.mean(cdev),
-                                  "subgraphs_per_kernel_s": c5 / (statistics.mean(cdev) / 1e3),
-                                  "alg_GBs": cst["alg_bytes"] / (statistics.mean(cdev) / 1e3) / 1e9,
-                                  "vwave": cst["vwave"]}
+        line["clique_rmat17"] = plain_vs_learned(gr, "clique", (5, 6), 4, args.warmup, args.steps)
+        line["learned_speedup"] = {"clique_k5_rmat17": line["clique_rmat17"]["k5"]["learned_speedup"],
+                                   "clique_k6_rmat17": line["clique_rmat17"]["k6"]["learned_speedup"]}
+    if args.full:
+        gb = W.ba_100k_planted()
+        line["motif5_ba"] = plain_vs_learned(gb, "motif", (5,), 4, 0, 1, budget_ms=2000.0)
+        line.setdefault("learned_speedup", {})["motif_k5_ba"] = line["motif5_ba"]["k5"]["learned_speedup"]
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(g, "motif", K, args.cpu_budget)
     print(json.dumps(line), flush=True)

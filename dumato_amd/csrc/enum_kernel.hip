@@ -688,19 +688,25 @@ struct Worker {
             maybe_donate(1, -1, dummy, 0);
             const int cur = R.cur, end = R.end;
             if (cur >= end) break;
-            const int take = end - cur < W ? end - cur : W;
-            R.cur = cur + take;
+            // a batch holds children until it has about `target` pairs (at
+            // least one child), so heavy roots reach a donation point often
+            const int avail = end - cur < W ? end - cur : W;
             int np = 0;
-            if (g.gl < take) {
-                const int i = cur + g.gl;
-                by += (unsigned long long)dp[i];
+            if (g.gl < avail) {
 #pragma unroll
-                for (int w = 0; w < NW; w++) np += __popcll(rows[i * NW + w]);
+                for (int w = 0; w < NW; w++) np += __popcll(rows[(cur + g.gl) * NW + w]);
             }
             const int sc = g.iscan(np);
-            if (g.gl < take) pre[g.gl + 1] = sc;
+            const int target = K == 4 ? 16 * W : 2 * W;
+            int take = __popcll(g.ballot(g.gl < avail && sc - np < target));
+            take = take < 1 ? 1 : take;
+            R.cur = cur + take;
+            if (g.gl < take) {
+                by += (unsigned long long)dp[cur + g.gl];
+                pre[g.gl + 1] = sc;
+            }
             if (g.gl == 0) pre[0] = 0;
-            const int P = g.bcast(sc, W - 1);
+            const int P = g.bcast(sc, take - 1);
             wsync();
             int a = 0;
             for (int t = g.gl; t < P; t += W) {

@@ -31,18 +31,11 @@ static double now_ms() {
     return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
 }
 
-// Width used when the caller asks for autotuning and no tuned value is
-// cached for (mode, k): a worker's lanes stride adjacency lists, so the width
-// follows the typical list length (rounded up to a power of two, 8..64).
+// Width used when no tuned value is cached for (mode, k): full 64-lane
+// waves, the fastest untuned choice on every benchmark graph measured.
 int autotuned_vwave(dm_graph *g, int K, bool motif) {
     int t = g->tuned_vwave[motif ? 1 : 0][K];
-    if (t) return t;
-    const HostCSR &c = g->rcsr;
-    double avg = c.n ? (double)c.nnz() / (double)c.n : 0.0;
-    if (!motif) avg *= 0.5;
-    int w = 8;
-    while (w < 64 && w < avg) w *= 2;
-    return w;
+    return t ? t : 64;
 }
 
 int tuned_steal_min(const dm_graph *g, int K, bool motif) {
@@ -81,45 +74,47 @@ int autotune(dm_graph *g, int K, bool motif, double budget_ms, dm_config *best_o
     dm_config base;
     dm_config_init(&base);
     base.steal_min = 2;
-    base.vwave = 32;
-    // size the sample: aim at budget / 12 per trial
-    int S = 64;
+    base.vwave = 64;
+    // roots with work, to size the sample: a trial must give every worker
+    // several jobs, or it times the heaviest sampled root instead of the
+    // throughput
+    int64_t nroots = 0;
+    for (int32_t v = 0; v < g->rcsr.n; v++) nroots += root_has_work(g, v, K, motif) ? 1 : 0;
+    Shape sh64;
+    if ((r = pick_shape(g, K, motif, base, sh64))) return r;
+    int S = (int)std::max<int64_t>(1, nroots / (8 * (int64_t)sh64.workers));
     double ms = 0;
     if ((r = trial(base, S, ms))) return r;
-    const double per = std::max(budget_ms / 12.0, 0.5);
-    while (ms > per && S < (1 << 20)) {
-        S *= 4;
-        if ((r = trial(base, S, ms))) return r;
-    }
-    while (ms * 4 < per && S > 1) {
-        S = std::max(1, S / 4);
-        if ((r = trial(base, S, ms))) return r;
-    }
-    double best = 1e300;
+    // too slow for the budget: fewer candidates rather than a thinner sample
+    const bool full_search = ms * 14 <= budget_ms;
+    const bool width_search = ms * 6 <= budget_ms;
+    double best = ms;
     dm_config bc = base;
-    for (int w : {8, 16, 32, 64}) {
-        dm_config c = base;
-        c.vwave = w;
-        double t1 = 0, t2 = 0;
-        if ((r = trial(c, S, t1)) || (r = trial(c, S, t2))) return r;
-        double t = std::min(t1, t2);
-        if (t < best) {
-            best = t;
-            bc = c;
-        }
-    }
-    const int dl = 32768 / (256 / bc.vwave);
-    for (int lds : {dl / 2, dl, dl * 2}) {
-        for (int sm : {2, 8}) {
-            dm_config c = bc;
-            c.lds_bytes = lds;
-            c.steal_min = sm;
-            double t1 = 0, t2 = 0;
-            if ((r = trial(c, S, t1)) || (r = trial(c, S, t2))) continue;  // e.g. LDS too large
-            double t = std::min(t1, t2);
+    if (width_search) {
+        for (int w : {8, 16, 32}) {
+            dm_config c = base;
+            c.vwave = w;
+            double t = 0;
+            if ((r = trial(c, S, t))) return r;
             if (t < best) {
                 best = t;
                 bc = c;
+            }
+        }
+    }
+    const int dl = 32768 / (256 / bc.vwave);
+    if (full_search) {
+        for (int lds : {dl / 2, dl}) {
+            for (int sm : {2, 8}) {
+                dm_config c = bc;
+                c.lds_bytes = lds;
+                c.steal_min = sm;
+                double t = 0;
+                if ((r = trial(c, S, t))) continue;
+                if (t < best) {
+                    best = t;
+                    bc = c;
+                }
             }
         }
     }
