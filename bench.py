@@ -119,6 +119,47 @@ def plain_vs_learned(graph, mode, k_list, k_train, warmup, steps, budget_ms=300.
     return out
 
 
+def batch_block(warmup, steps, check_graphs=8):
+    """256 seeded graphs x {clique k=3..6, motif k=3..5} in one hipGraph-replayed launch."""
+    import numpy as np
+
+    import dumato_amd as dm
+    from dumato_amd import workloads as W
+    t = time.perf_counter()
+    specs = W.batch_specs(256)
+    graphs = [W.make_batch_graph(s) for s in specs]
+    queries = [(gi, mode, k) for gi in range(len(graphs)) for (mode, k) in W.BATCH_QUERIES]
+    b = dm.Batch(graphs)
+    gen_s = time.perf_counter() - t
+    best = b.autotune(queries, budget_ms=3000.0)
+    for _ in range(warmup):
+        b.run(queries)
+    kms, lats = [], []
+    res = None
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        res, lat = b.run(queries)
+        kms.append(b.last_stats()["device_ms"])
+        lats.append(lat)
+    wall = time.perf_counter() - t0
+    total = sum(int(np.asarray(r).sum()) for r in res)
+    # spot-check: the first graphs' results against the single-graph path
+    ok = True
+    for gi in range(check_graphs):
+        for (mode, k), r in zip(W.BATCH_QUERIES, res[gi * len(W.BATCH_QUERIES):(gi + 1) * len(W.BATCH_QUERIES)]):
+            one = graphs[gi].motif_count(k)[1] if mode == "motif" else graphs[gi].clique_count(k)
+            ok = ok and np.array_equal(np.asarray(one), np.asarray(r))
+    km = statistics.median(kms)
+    lat = np.median(np.stack(lats), axis=0)
+    st = b.last_stats()
+    return {"graphs": len(graphs), "queries": len(queries), "subgraphs": total,
+            "kernel_ms": km, "subgraphs_per_s": total / (km / 1e3), "wall_ms_per_step": 1e3 * wall / steps,
+            "latency_ms_p50": float(np.percentile(lat, 50)), "latency_ms_p99": float(np.percentile(lat, 99)),
+            "vwave": st["vwave"], "workers": st["workers"], "steal_min": best.steal_min,
+            "donations": st["donations"], "graph_build_s": gen_s,
+            "spot_check_vs_single_graph_path": ok, "hipgraph": True}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -128,6 +169,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--full", action="store_true", help="also the k=5 motif run on the BA graph (~1 min)")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 256-graph batched launch")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -184,6 +226,8 @@ def main():
         line["clique_rmat17"] = plain_vs_learned(gr, "clique", (5, 6), 4, args.warmup, args.steps)
         line["learned_speedup"] = {"clique_k5_rmat17": line["clique_rmat17"]["k5"]["learned_speedup"],
                                    "clique_k6_rmat17": line["clique_rmat17"]["k6"]["learned_speedup"]}
+    if not args.no_batch:
+        line["batch256"] = batch_block(args.warmup, max(3, args.steps // 2))
     if args.full:
         gb = W.ba_100k_planted()
         line["motif5_ba"] = plain_vs_learned(gb, "motif", (5,), 4, 0, 1, budget_ms=2000.0)

@@ -87,6 +87,14 @@ SYMBOLS = [
                                       ctypes.POINTER(ctypes.c_double)]),
     ("dm_sched_predict", ctypes.c_int, [GP, f32p, ctypes.c_size_t]),
     ("dm_autotune", ctypes.c_int, [GP, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.POINTER(Config)]),
+    ("dm_batch_create", ctypes.c_int, [ctypes.POINTER(GP), ctypes.c_size_t, ctypes.POINTER(GP)]),
+    ("dm_batch_num_results", ctypes.c_int, [GP, ctypes.c_void_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]),
+    ("dm_batch_run", ctypes.c_int, [GP, ctypes.c_void_p, ctypes.c_size_t, ctypes.POINTER(Config), u64p,
+                                    ctypes.c_size_t, u64p, ctypes.POINTER(ctypes.c_double)]),
+    ("dm_batch_autotune", ctypes.c_int, [GP, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_double,
+                                         ctypes.POINTER(Config)]),
+    ("dm_batch_last_stats", ctypes.c_int, [GP, ctypes.POINTER(Stats)]),
+    ("dm_batch_free", None, [GP]),
     ("dm_mlp_grad", ctypes.c_int, [f32p, f32p, f32p, ctypes.c_int, f32p, f32p, f32p]),
     ("dm_mlp_adam", ctypes.c_int, [f32p, f32p, f32p, ctypes.c_int, ctypes.c_int, ctypes.c_float, f32p]),
 ]
@@ -283,3 +291,61 @@ def mlp_adam(params, X, Y, steps, lr):
     _check(lib().dm_mlp_adam(_ptr(P, f32p), _ptr(X, f32p), _ptr(Y, f32p), len(Y), int(steps),
                              float(lr), _ptr(losses, f32p)))
     return P, losses
+
+
+class Query(ctypes.Structure):
+    """Mirror of struct dm_query."""
+    _fields_ = [("graph", ctypes.c_int32), ("motif", ctypes.c_int32), ("k", ctypes.c_int32),
+                ("reserved", ctypes.c_int32)]
+
+
+class Batch:
+    """Many graphs and (pattern, k) queries in one persistent launch (hipGraph-replayed)."""
+
+    def __init__(self, graphs):
+        arr = (GP * len(graphs))(*[g._h.value for g in graphs])
+        h = GP()
+        _check(lib().dm_batch_create(arr, len(graphs), ctypes.byref(h)))
+        self._h = h
+        self.ngraphs = len(graphs)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value and _lib is not None:
+            _lib.dm_batch_free(h)
+            self._h = None
+
+    @staticmethod
+    def _queries(queries):
+        q = (Query * len(queries))()
+        for i, (g, mode, k) in enumerate(queries):
+            q[i].graph, q[i].motif, q[i].k = int(g), 1 if mode == "motif" else 0, int(k)
+        return q
+
+    def run(self, queries, cfg=None):
+        """queries: [(graph index, "clique"|"motif", k)].  Returns (list of results
+        -- int for cliques, uint64 histogram for motifs --, per-graph latency ms)."""
+        q = self._queries(queries)
+        nres = ctypes.c_size_t()
+        _check(lib().dm_batch_num_results(self._h, q, len(queries), ctypes.byref(nres)))
+        res = np.zeros(max(nres.value, 1), np.uint64)
+        offs = np.zeros(len(queries) + 1, np.uint64)
+        lat = (ctypes.c_double * self.ngraphs)()
+        _check(lib().dm_batch_run(self._h, q, len(queries), ctypes.byref(cfg) if cfg else None,
+                                  _ptr(res, u64p), len(res), _ptr(offs, u64p), lat))
+        out = []
+        for i, (_, mode, _) in enumerate(queries):
+            a, b = int(offs[i]), int(offs[i + 1])
+            out.append(res[a:b].copy() if mode == "motif" else int(res[a]))
+        return out, np.array(lat[:], dtype=np.float64)
+
+    def autotune(self, queries, budget_ms=2000.0):
+        q = self._queries(queries)
+        c = Config()
+        _check(lib().dm_batch_autotune(self._h, q, len(queries), float(budget_ms), ctypes.byref(c)))
+        return c
+
+    def last_stats(self):
+        s = Stats()
+        _check(lib().dm_batch_last_stats(self._h, ctypes.byref(s)))
+        return s.as_dict()

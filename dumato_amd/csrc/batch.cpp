@@ -1,0 +1,460 @@
+// Batched queries over many graphs in one persistent launch, captured once in
+// a hipGraph and replayed.  No reference counterpart: DuMato runs one graph
+// and one k per executable (src/main/clique_PA.cu:34-130).
+//
+// The graphs are concatenated into one disjoint-union CSR (each relabelled by
+// degree rank inside its own id range, so cliques and motifs never cross
+// graphs); every (graph, pattern, k) query gets a descriptor with its class
+// table and result slots, and static range jobs over its graph's roots.  The
+// kernel switches query per job; donation and counting are the single-graph
+// kernel's.
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <chrono>
+#include <climits>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <tuple>
+
+#include "internal.h"
+#include "runtime.h"
+
+namespace dm {
+
+#define HIPCHK(x)                                                                                    \
+    do {                                                                                             \
+        hipError_t e_ = (x);                                                                         \
+        if (e_ != hipSuccess) return set_error(DM_ERR_HIP, "%s: %s", #x, hipGetErrorString(e_));     \
+    } while (0)
+
+struct BatchPlan {
+    std::vector<dm_query> queries;
+    int vwave = 0, lds = 0, steal = 1, steal_min = 2, steal_period = 8, bitmap = 1, blocks = 0;
+    double watchdog_s = 0;
+    bool same(const std::vector<dm_query> &q, const dm_config &c, int vw, int l, int sm) const {
+        if (q.size() != queries.size()) return false;
+        for (size_t i = 0; i < q.size(); i++)
+            if (q[i].graph != queries[i].graph || q[i].motif != queries[i].motif || q[i].k != queries[i].k)
+                return false;
+        return vw == vwave && l == lds && c.steal == steal && sm == steal_min && c.steal_period == steal_period &&
+               c.clique_bitmap == bitmap && c.blocks == blocks && c.watchdog_s == watchdog_s;
+    }
+};
+
+}  // namespace dm
+
+struct dm_batch {
+    int ngraphs = 0;
+    std::vector<int32_t> vbase;      // first union id of each graph
+    std::vector<int32_t> gn;         // vertices per graph
+    dm::HostCSR u;                   // union CSR (degree-rank ids per graph)
+    std::vector<int32_t> ostart;
+    int32_t max_out = 0;
+    // device
+    dm::DeviceGraph *dev = nullptr;
+    dm::QueryDesc *d_qd = nullptr;
+    size_t qd_cap = 0;
+    unsigned long long *d_done = nullptr;
+    size_t done_cap = 0;
+    unsigned long long *d_start = nullptr;
+    size_t start_cap = 0;
+    // cached launch
+    dm::BatchPlan plan;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    size_t nres = 0;
+    std::vector<uint64_t> res_off;
+    std::vector<int32_t> q_graph;
+    dm_stats last{};
+};
+
+namespace dm {
+
+static double now_ms() {
+    using namespace std::chrono;
+    return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
+}
+
+static void drop_graph(dm_batch *b) {
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+}
+
+static int batch_device(dm_batch *b) {
+    if (b->dev) return DM_OK;
+    // reuse the single-graph device setup on a stand-in handle of the union
+    dm_graph tmp;
+    tmp.rcsr = b->u;
+    tmp.ostart = b->ostart;
+    tmp.max_out = b->max_out;
+    int r = ensure_device(&tmp);
+    b->dev = tmp.dev;
+    tmp.dev = nullptr;
+    return r;
+}
+
+// Range jobs: each query's roots in chunks; a chunk's cost estimate is the
+// sum over its roots of (out-degree)^(k-2) (cliques) or degree^(k-2)
+// (motifs).  Dealt heaviest first, round-robin.
+static void batch_partition(const dm_batch *b, const std::vector<dm_query> &qs, int workers,
+                            std::vector<int64_t> &sq_off, std::vector<int32_t> &sq_jobs) {
+    struct Piece {
+        double cost;
+        int32_t r0, r1, q;
+    };
+    std::vector<Piece> pieces;
+    const int chunk = 32;
+    for (size_t qi = 0; qi < qs.size(); qi++) {
+        const dm_query &Q = qs[qi];
+        const int32_t v0 = b->vbase[Q.graph], v1 = v0 + b->gn[Q.graph];
+        for (int32_t r = v0; r < v1; r += chunk) {
+            const int32_t e = std::min(r + chunk, v1);
+            double c = 0;
+            for (int32_t v = r; v < e; v++) {
+                double d = Q.motif ? (double)(b->u.off[v + 1] - b->u.off[v]) : (double)(b->u.off[v + 1] - b->ostart[v]);
+                c += std::pow(d, std::max(1, Q.k - 2));
+            }
+            if (c > 0) pieces.push_back({c, r, e, (int32_t)qi});
+        }
+    }
+    std::sort(pieces.begin(), pieces.end(), [](const Piece &a, const Piece &c) { return a.cost > c.cost; });
+    sq_off.assign((size_t)workers + 1, 0);
+    const int64_t n = (int64_t)pieces.size();
+    for (int w = 0; w < workers; w++) sq_off[w + 1] = sq_off[w] + (n > w ? (n - w + workers - 1) / workers : 0);
+    sq_jobs.assign((size_t)n * 4, 0);
+    for (int w = 0; w < workers; w++) {
+        int64_t o = sq_off[w];
+        for (int64_t i = w; i < n; i += workers, o++) {
+            sq_jobs[4 * o] = pieces[i].r0;
+            sq_jobs[4 * o + 1] = -1;
+            sq_jobs[4 * o + 2] = pieces[i].r1;
+            sq_jobs[4 * o + 3] = pieces[i].q;
+        }
+    }
+}
+
+static int batch_build(dm_batch *b, const std::vector<dm_query> &qs, const dm_config &cfg) {
+    DeviceGraph *d = b->dev;
+    int kmax = 2, hslots = 0, mode = -1;
+    for (const dm_query &Q : qs) {
+        if (Q.graph < 0 || Q.graph >= b->ngraphs) return set_error(DM_ERR_ARG, "query graph %d out of range", Q.graph);
+        if (Q.motif) {
+            if (Q.k < 3 || Q.k > 6) return set_error(DM_ERR_UNSUPPORTED, "batch motif k=%d (3..6)", Q.k);
+            const PatternTable *t = pattern_table(Q.k);
+            if (!t) return last_code();
+            hslots = std::max(hslots, t->ncls);
+        } else if (Q.k < 2 || Q.k > kMaxK) {
+            return set_error(DM_ERR_ARG, "batch clique k=%d (2..%d)", Q.k, kMaxK);
+        }
+        kmax = std::max(kmax, Q.k);
+        int m = Q.motif ? 1 : 0;
+        mode = mode < 0 ? m : (mode == m ? m : 2);
+    }
+    if (qs.size() == 1) mode = 2;  // one query still uses the batch counters
+    // launch shape: width / LDS from the config (autotuned values cached in the plan)
+    int vw = cfg.vwave ? cfg.vwave : (b->plan.vwave ? b->plan.vwave : 64);
+    if (vw != 8 && vw != 16 && vw != 32 && vw != 64) return set_error(DM_ERR_ARG, "vwave must be 8/16/32/64");
+    int lds = cfg.lds_bytes > 0 ? cfg.lds_bytes : 32768 / (256 / vw);
+    lds = (lds + 15) & ~15;
+    const int sm = cfg.steal_min ? cfg.steal_min : 2;
+    if (b->exec && b->plan.same(qs, cfg, vw, lds, sm)) return DM_OK;
+    drop_graph(b);
+    // one query: class counters shared per block; several: per worker
+    const int ncls_single = qs.size() == 1 && qs[0].motif ? hslots : 0;
+    const int wslots = qs.size() > 1 ? hslots : 0;
+    size_t lds_total = enum_lds_bytes(vw, 256, kmax, ncls_single, wslots, lds);
+    int bpc = 0;
+    int r = enum_occupancy(vw, mode, 256, lds_total, &bpc);
+    if (r) return r;
+    if (bpc < 1) return set_error(DM_ERR_ARG, "batch launch does not fit one CU (LDS %zu B)", lds_total);
+    const int blocks = cfg.blocks > 0 ? cfg.blocks : bpc * d->num_cu;
+    const int workers = blocks * (256 / vw);
+    std::vector<int64_t> sq_off;
+    std::vector<int32_t> sq_jobs;
+    batch_partition(b, qs, workers, sq_off, sq_jobs);
+    const int64_t njobs = (int64_t)sq_jobs.size() / 4;
+    // result slots
+    b->res_off.assign(qs.size() + 1, 0);
+    for (size_t i = 0; i < qs.size(); i++)
+        b->res_off[i + 1] = b->res_off[i] + (qs[i].motif ? (uint64_t)pattern_table(qs[i].k)->ncls : 1);
+    b->nres = b->res_off.back();
+    int64_t arena = 256;
+    for (const dm_query &Q : qs)
+        arena = std::max(arena, arena_bound_for(b->u.maxdeg, b->max_out, Q.k, Q.motif != 0));
+    if ((r = dev_grow(d->d_arena, d->arena_cap, (size_t)arena * workers))) return r;
+    if ((r = dev_grow(d->d_sq_off, d->sq_off_cap, sq_off.size()))) return r;
+    if ((r = dev_grow(d->d_sq_jobs, d->sq_jobs_cap, std::max<size_t>(sq_jobs.size(), 4)))) return r;
+    if ((r = dev_grow(d->d_sq_cur, d->sq_cur_cap, (size_t)workers))) return r;
+    if ((r = ensure_queue(d))) return r;
+    if ((r = dev_grow(d->d_results, d->results_cap, b->nres))) return r;
+    if ((r = dev_grow(b->d_qd, b->qd_cap, qs.size()))) return r;
+    if ((r = dev_grow(b->d_done, b->done_cap, qs.size()))) return r;
+    if ((r = dev_grow(b->d_start, b->start_cap, 1))) return r;
+    std::vector<QueryDesc> qd(qs.size());
+    b->q_graph.assign(qs.size(), 0);
+    for (size_t i = 0; i < qs.size(); i++) {
+        const dm_query &Q = qs[i];
+        QueryDesc &D = qd[i];
+        memset(&D, 0, sizeof(D));
+        D.K = Q.k;
+        D.motif = Q.motif ? 1 : 0;
+        D.graph = Q.graph;
+        b->q_graph[i] = Q.graph;
+        if (Q.motif) {
+            const PatternTable *t = pattern_table(Q.k);
+            D.ncls = t->ncls;
+            if (!d->d_lut[Q.k]) {
+                size_t c = 0;
+                if ((r = dev_grow(d->d_lut[Q.k], c, t->q2c.size()))) return r;
+                HIPCHK(hipMemcpy(d->d_lut[Q.k], t->q2c.data(), t->q2c.size() * 2, hipMemcpyHostToDevice));
+            }
+            D.lut = d->d_lut[Q.k];
+        }
+        D.out = d->d_results + b->res_off[i];
+        D.done_ticks = b->d_done + i;
+    }
+    HIPCHK(hipMemcpy(b->d_qd, qd.data(), qd.size() * sizeof(QueryDesc), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(d->d_sq_off, sq_off.data(), sq_off.size() * 8, hipMemcpyHostToDevice));
+    if (!sq_jobs.empty()) HIPCHK(hipMemcpy(d->d_sq_jobs, sq_jobs.data(), sq_jobs.size() * 4, hipMemcpyHostToDevice));
+
+    EnumParams p;
+    memset(&p, 0, sizeof(p));
+    p.off = d->d_off;
+    p.adj = d->d_adj;
+    p.ostart = d->d_ostart;
+    p.n = b->u.n;
+    p.nq = (int32_t)qs.size();
+    p.qd = b->d_qd;
+    p.kmax = kmax;
+    p.hist_slots = wslots;
+    p.nworkers = workers;
+    p.sq_off = d->d_sq_off;
+    p.sq_jobs = d->d_sq_jobs;
+    p.sq_cur = d->d_sq_cur;
+    p.q = d->d_q;
+    p.q_cap = (uint32_t)d->q_cap_alloc;
+    p.epoch = d->d_epoch;
+    p.start_ticks = b->d_start;
+    p.sc = d->d_sc;
+    p.arena = d->d_arena;
+    p.arena_per_worker = arena;
+    p.lds_per_worker = lds;
+    p.steal = cfg.steal ? 1 : 0;
+    p.steal_min = sm;
+    p.steal_period = cfg.steal_period > 0 ? cfg.steal_period : 8;
+    p.bitmap = cfg.clique_bitmap ? 1 : 0;
+    p.watchdog_ticks = cfg.watchdog_s > 0 ? (uint64_t)(cfg.watchdog_s * d->wall_khz * 1000.0) : 0;
+    // capture: init kernel + persistent kernel
+    hipStream_t s = d->stream;
+    HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    int rr = enum_init_launch(p, (unsigned long long)njobs, d->d_epoch, d->d_results, b->nres, b->d_done,
+                              (int)qs.size(), s);
+    if (!rr && njobs > 0) rr = enum_launch(p, mode, vw, blocks, 256, lds_total, s);
+    hipError_t ce = hipStreamEndCapture(s, &b->graph);
+    if (rr) return rr;
+    if (ce != hipSuccess) return set_error(DM_ERR_HIP, "graph capture: %s", hipGetErrorString(ce));
+    HIPCHK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    b->plan.queries = qs;
+    b->plan.vwave = vw;
+    b->plan.lds = lds;
+    b->plan.steal = cfg.steal;
+    b->plan.steal_min = sm;
+    b->plan.steal_period = cfg.steal_period;
+    b->plan.bitmap = cfg.clique_bitmap;
+    b->plan.blocks = cfg.blocks;
+    b->plan.watchdog_s = cfg.watchdog_s;
+    b->last.vwave = vw;
+    b->last.workers = workers;
+    b->last.blocks = blocks;
+    b->last.lds_bytes = lds;
+    return DM_OK;
+}
+
+static int batch_run(dm_batch *b, const std::vector<dm_query> &qs, const dm_config &cfg, uint64_t *results,
+                     size_t results_len, uint64_t *offsets, double *latency_ms) {
+    const double t0 = now_ms();
+    int r = batch_device(b);
+    if (r) return r;
+    if ((r = batch_build(b, qs, cfg))) return r;
+    const double t1 = now_ms();
+    DeviceGraph *d = b->dev;
+    hipStream_t s = d->stream;
+    if (results && results_len < b->nres) return set_error(DM_ERR_ARG, "results need %zu slots", b->nres);
+    HIPCHK(hipEventRecord(d->ev0, s));
+    HIPCHK(hipGraphLaunch(b->exec, s));
+    HIPCHK(hipEventRecord(d->ev1, s));
+    SchedCounters sc1;
+    HIPCHK(hipMemcpyAsync(&sc1, d->d_sc, sizeof(sc1), hipMemcpyDeviceToHost, s));
+    std::vector<unsigned long long> res(b->nres), done(qs.size());
+    unsigned long long start = 0;
+    HIPCHK(hipMemcpyAsync(res.data(), d->d_results, b->nres * 8, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(done.data(), b->d_done, qs.size() * 8, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(&start, b->d_start, 8, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    float kms = 0;
+    HIPCHK(hipEventElapsedTime(&kms, d->ev0, d->ev1));
+    if (sc1.error) {
+        const char *why = sc1.error == 1 ? "watchdog expired" : sc1.error == 2 ? "stack arena overflow" : "invalid job";
+        return set_error(DM_ERR_DEVICE, "batch kernel aborted: %s", why);
+    }
+    uint64_t total = 0;
+    for (size_t i = 0; i < b->nres; i++) total += res[i];
+    if (results)
+        for (size_t i = 0; i < b->nres; i++) results[i] = res[i];
+    if (offsets)
+        for (size_t i = 0; i <= qs.size(); i++) offsets[i] = b->res_off[i];
+    if (latency_ms) {
+        for (int gi = 0; gi < b->ngraphs; gi++) latency_ms[gi] = 0;
+        for (size_t i = 0; i < qs.size(); i++) {
+            double ms = done[i] > start ? (double)(done[i] - start) / d->wall_khz : 0.0;
+            latency_ms[b->q_graph[i]] = std::max(latency_ms[b->q_graph[i]], ms);
+        }
+    }
+    dm_stats &st = b->last;
+    st.device_ms = kms;
+    st.sched_ms = t1 - t0;
+    st.wall_ms = now_ms() - t0;
+    st.subgraphs = total;
+    st.alg_bytes = sc1.alg_bytes;
+    st.donations = sc1.donations;
+    st.jobs = sc1.jobs;
+    st.busy_sum_cycles = sc1.busy_sum;
+    st.busy_max_cycles = sc1.busy_max;
+    return DM_OK;
+}
+
+}  // namespace dm
+
+using namespace dm;
+
+extern "C" {
+
+int dm_batch_create(dm_graph *const *graphs, size_t ngraphs, dm_batch **out) {
+    if (!graphs || !out || ngraphs == 0) return set_error(DM_ERR_ARG, "need graphs and out");
+    *out = nullptr;
+    dm_batch *b = new dm_batch();
+    b->ngraphs = (int)ngraphs;
+    int64_t nv = 0, ne = 0;
+    for (size_t i = 0; i < ngraphs; i++) {
+        if (!graphs[i]) {
+            delete b;
+            return set_error(DM_ERR_ARG, "graph %zu is NULL", i);
+        }
+        nv += graphs[i]->rcsr.n;
+        ne += graphs[i]->rcsr.nnz();
+    }
+    if (nv > INT_MAX - 1) {
+        delete b;
+        return set_error(DM_ERR_ARG, "batch has too many vertices");
+    }
+    b->u.n = (int32_t)nv;
+    b->u.off.assign((size_t)nv + 1, 0);
+    b->u.adj.resize((size_t)ne);
+    b->ostart.assign((size_t)nv, 0);
+    int32_t vb = 0;
+    int64_t eb = 0;
+    for (size_t i = 0; i < ngraphs; i++) {
+        const dm_graph *g = graphs[i];
+        b->vbase.push_back(vb);
+        b->gn.push_back(g->rcsr.n);
+        for (int32_t v = 0; v < g->rcsr.n; v++) {
+            b->u.off[vb + v + 1] = eb + g->rcsr.off[v + 1];
+            b->ostart[vb + v] = (int32_t)(eb + g->ostart[v]);
+        }
+        for (int64_t e = 0; e < g->rcsr.nnz(); e++) b->u.adj[eb + e] = vb + g->rcsr.adj[e];
+        b->u.maxdeg = std::max(b->u.maxdeg, g->rcsr.maxdeg);
+        b->max_out = std::max(b->max_out, g->max_out);
+        vb += g->rcsr.n;
+        eb += g->rcsr.nnz();
+    }
+    *out = b;
+    return DM_OK;
+}
+
+int dm_batch_run(dm_batch *b, const dm_query *queries, size_t nq, const dm_config *cfg, uint64_t *results,
+                 size_t results_len, uint64_t *result_offsets, double *graph_latency_ms) {
+    if (!b || !queries || nq == 0) return set_error(DM_ERR_ARG, "need batch and queries");
+    dm_config c;
+    if (cfg) c = *cfg;
+    else dm_config_init(&c);
+    std::vector<dm_query> qs(queries, queries + nq);
+    return batch_run(b, qs, c, results, results_len, result_offsets, graph_latency_ms);
+}
+
+int dm_batch_autotune(dm_batch *b, const dm_query *queries, size_t nq, double budget_ms, dm_config *best) {
+    if (!b || !queries || nq == 0) return set_error(DM_ERR_ARG, "need batch and queries");
+    std::vector<dm_query> qs(queries, queries + nq);
+    dm_config base;
+    dm_config_init(&base);
+    double t_best = 1e300;
+    dm_config bc = base;
+    const double t_start = now_ms();
+    for (int w : {64, 32, 16}) {
+        for (int sm : {2, 8}) {
+            dm_config c = base;
+            c.vwave = w;
+            c.steal_min = sm;
+            int r = batch_run(b, qs, c, nullptr, 0, nullptr, nullptr);
+            if (r) return r;
+            r = batch_run(b, qs, c, nullptr, 0, nullptr, nullptr);  // replay timing
+            if (r) return r;
+            if (b->last.device_ms < t_best) {
+                t_best = b->last.device_ms;
+                bc = c;
+            }
+            if (now_ms() - t_start > budget_ms) break;
+        }
+        if (now_ms() - t_start > budget_ms) break;
+    }
+    bc.lds_bytes = 32768 / (256 / bc.vwave);
+    b->plan.vwave = bc.vwave;
+    if (best) *best = bc;
+    return DM_OK;
+}
+
+int dm_batch_last_stats(const dm_batch *b, dm_stats *st) {
+    if (!b || !st) return set_error(DM_ERR_ARG, "NULL argument");
+    *st = b->last;
+    return DM_OK;
+}
+
+int dm_batch_num_results(const dm_batch *b, const dm_query *queries, size_t nq, size_t *nres) {
+    if (!b || !queries || !nres) return set_error(DM_ERR_ARG, "NULL argument");
+    size_t t = 0;
+    for (size_t i = 0; i < nq; i++) {
+        if (queries[i].motif) {
+            const PatternTable *p = pattern_table(queries[i].k);
+            if (!p) return last_code();
+            t += (size_t)p->ncls;
+        } else {
+            t += 1;
+        }
+    }
+    *nres = t;
+    return DM_OK;
+}
+
+void dm_batch_free(dm_batch *b) {
+    if (!b) return;
+    drop_graph(b);
+    if (b->d_qd) (void)hipFree(b->d_qd);
+    if (b->d_done) (void)hipFree(b->d_done);
+    if (b->d_start) (void)hipFree(b->d_start);
+    if (b->dev) {
+        dm_graph tmp;
+        tmp.dev = b->dev;
+        b->dev = nullptr;
+        // graph handle destructor path frees the device buffers
+        dm_graph *h = new dm_graph();
+        h->dev = tmp.dev;
+        tmp.dev = nullptr;
+        dm_graph_free(h);
+    }
+    delete b;
+}
+
+}  // extern "C"

@@ -34,6 +34,8 @@
 //     instead of by the host).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "internal.h"
 
 namespace dm {
@@ -146,12 +148,16 @@ DEV WorkerLDS worker_lds(char *base, int K) {
     return w;
 }
 
-template <int W, bool MOTIF>
+template <int W, int MODE>
 struct Worker {
     Grp<W> g;
     const EnumParams &p;
     WorkerLDS ws;
-    unsigned long long *shist;
+    unsigned long long *shist;       // class counters the leaf phase adds to
+    const uint16_t *lut;
+    int qid;
+    bool qmotif;
+    uint32_t epoch;
     char *lds_lo, *lds_top, *lds_hi;
     char *g_lo, *g_top, *g_hi;
     int K;
@@ -171,7 +177,11 @@ struct Worker {
         lds_hi = lds_slice + p.lds_per_worker;
         g_lo = g_top = p.arena + (int64_t)wid * p.arena_per_worker;
         g_hi = g_lo + p.arena_per_worker;
-        K = p.K;
+        K = 0;
+        qid = -1;
+        qmotif = MODE == 1;
+        lut = nullptr;
+        epoch = (uint32_t)*p.epoch;
         steps = 0;
         cnt = bytes = donated = 0;
         abort = false;
@@ -245,13 +255,13 @@ struct Worker {
         if (!ok) return false;
         slot = (uint32_t)g.bcast((int)slot, 0);
         uint64_t *dst = p.q[slot].w;
-        const uint64_t tag = (uint64_t)(uint32_t)p.epoch << 32;
+        const uint64_t tag = (uint64_t)epoch << 32;
         for (int j = g.gl; j < kJobWords; j += W) {
             int32_t v;
             if (j == 0) v = lvl;
             else if (j == 1) v = lo;
             else if (j == 2) v = hi;
-            else if (j == 3) v = 0;
+            else if (j == 3) v = qid;
             else v = (j - 4 <= lvl) ? ws.lv[j - 4].vtx : -1;
             __hip_atomic_store(dst + j, tag | (uint32_t)v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
@@ -453,12 +463,12 @@ struct Worker {
             for (int j = g.gl; j < npm; j += W) {
                 uint32_t a = ws.n_int[j];
                 uint32_t b = ws.n_gt[j] - a;
-                if (a) atomicAdd(&shist[p.lut[indc + ((uint64_t)(j | top) << s)]], (unsigned long long)a);
-                if (b) atomicAdd(&shist[p.lut[indc + ((uint64_t)j << s)]], (unsigned long long)b);
+                if (a) atomicAdd(&shist[lut[indc + ((uint64_t)(j | top) << s)]], (unsigned long long)a);
+                if (b) atomicAdd(&shist[lut[indc + ((uint64_t)j << s)]], (unsigned long long)b);
                 ws.n_int[j] = 0;
             }
             if (g.gl == 0) {
-                if (n_new) atomicAdd(&shist[p.lut[indc + ((uint64_t)top << s)]], (unsigned long long)n_new);
+                if (n_new) atomicAdd(&shist[lut[indc + ((uint64_t)top << s)]], (unsigned long long)n_new);
                 ws.n_gt[pmc] += 1;
                 const unsigned long long leaves = (unsigned long long)n_new + (unsigned long long)gt_total;
                 cnt += leaves;
@@ -743,7 +753,7 @@ struct Worker {
         lds_top = lds_lo;
         g_top = g_lo;
         cnt = 0;
-        constexpr bool motif = MOTIF;
+        const bool motif = MODE == 2 ? qmotif : MODE == 1;
         if (!motif && K == 2) {
             if (g.gl == 0) {
                 int d = odeg(v0);
@@ -837,27 +847,29 @@ struct Worker {
 };
 
 // ---------------------------------------------------------------------------
-// work acquisition.  A job head (lvl, lo, hi) comes back in registers; its
-// prefix ids go to the worker's LDS record (jid).
+// work acquisition.  A job head comes back in registers; its prefix ids go to
+// the worker's LDS record (jid).
 struct JobHead {
-    int lvl, lo, hi;
+    int lvl, lo, hi, q;
+    int range_end;  // > 0: a static range job over roots [jid[0], range_end)
     bool donated;
 };
 
 template <int W>
 DEV bool take_static(const EnumParams &p, const Grp<W> &g, int w, const WorkerLDS &ws, JobHead &jh) {
     int64_t b = p.sq_off[w], e = p.sq_off[w + 1];
-    int got = 0, r = 0, lo = 0, hi = 0;
+    int got = 0, r = 0, lo = 0, hi = 0, q = 0;
     if (g.gl == 0 && b < e) {
         int cur = __hip_atomic_load(&p.sq_cur[w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (b + cur < e) {
             int j = atomicAdd(&p.sq_cur[w], 1);
             if (b + j < e) {
                 got = 1;
-                const int32_t *t = p.sq_jobs + 3 * (b + j);
+                const int32_t *t = p.sq_jobs + 4 * (b + j);
                 r = t[0];
                 lo = t[1];
                 hi = t[2];
+                q = t[3];
                 atomicSub(&p.sc->static_left, 1);
             }
         }
@@ -866,9 +878,19 @@ DEV bool take_static(const EnumParams &p, const Grp<W> &g, int w, const WorkerLD
     if (!got) return false;
     jh.lvl = 0;
     jh.donated = false;
-    jh.lo = g.bcast(lo, 0);
-    jh.hi = g.bcast(hi, 0);
+    lo = g.bcast(lo, 0);
+    hi = g.bcast(hi, 0);
+    jh.q = g.bcast(q, 0);
     r = g.bcast(r, 0);
+    if (lo < 0) {  // range job
+        jh.range_end = hi;
+        jh.lo = 0;
+        jh.hi = 0x7fffffff;
+    } else {
+        jh.range_end = 0;
+        jh.lo = lo;
+        jh.hi = hi;
+    }
     ws.jid[0] = r;
     wsync();
     return true;
@@ -881,7 +903,7 @@ DEV bool take_static(const EnumParams &p, const Grp<W> &g, int w, const WorkerLD
 // has arrived (and clears the ticket).
 template <int W>
 DEV bool poll_ticket(const EnumParams &p, const Grp<W> &g, const WorkerLDS &ws, JobHead &jh, bool &has_ticket,
-                     uint32_t &ticket) {
+                     uint32_t &ticket, uint32_t ep) {
     if (!has_ticket) {
         uint32_t t = 0;
         if (g.gl == 0) t = atomicAdd(&p.sc->q_head, 1u);
@@ -890,7 +912,6 @@ DEV bool poll_ticket(const EnumParams &p, const Grp<W> &g, const WorkerLDS &ws, 
     }
     if (ticket >= p.q_cap) return false;  // no slot: wait for the end
     const uint64_t *src = p.q[ticket].w;
-    const uint32_t ep = (uint32_t)p.epoch;
     int v[(kJobWords + W - 1) / W];
     bool mine_ok = true;
 #pragma unroll
@@ -912,6 +933,8 @@ DEV bool poll_ticket(const EnumParams &p, const Grp<W> &g, const WorkerLDS &ws, 
     jh.lvl = g.bcast(v[0], 0);
     jh.lo = g.bcast(v[0], 1);
     jh.hi = g.bcast(v[0], 2);
+    jh.q = g.bcast(v[0], 3);
+    jh.range_end = 0;
     jh.donated = true;
     has_ticket = false;
     wsync();
@@ -936,24 +959,55 @@ DEV bool steal_static(const EnumParams &p, const Grp<W> &g, int w, const WorkerL
     return false;
 }
 
-template <int W, bool MOTIF>
-__global__ __launch_bounds__(kThreads, MOTIF ? 4 : 3) void enum_kernel(EnumParams p) {
+// Adds a worker's counts for query `q` to the query's result: the clique count
+// (one lane's total) and, in batch launches, the worker's class counters.
+template <int W>
+DEV void flush_query(const EnumParams &p, const Grp<W> &g, int q, unsigned long long count,
+                     unsigned long long *whist) {
+    if (q < 0) return;
+    const QueryDesc &Q = p.qd[q];
+    if (g.gl == 0 && count && !Q.motif) atomicAdd(&Q.out[0], count);
+    if (p.nq > 1 && Q.motif) {
+        for (int i = g.gl; i < Q.ncls; i += W) {
+            unsigned long long v = whist[i];
+            if (v) {
+                atomicAdd(&Q.out[i], v);
+                whist[i] = 0;
+            }
+        }
+        wsync();
+    }
+}
+
+// MODE 0: cliques only, 1: motifs only, 2: mixed (batch launches).
+template <int W, int MODE>
+__global__ __launch_bounds__(kThreads, MODE == 1 ? 4 : 3) void enum_kernel(EnumParams p) {
     extern __shared__ __align__(16) char smem[];
     Grp<W> g;
     const int wpb = kThreads / W;
     const int wib = threadIdx.x / W;
     const int wid = blockIdx.x * wpb + wib;
-    const int ncls_al = (p.ncls + 1) & ~1;
-    const size_t wsb = worker_lds_bytes(p.K);
-    unsigned long long *shist = (unsigned long long *)smem;
-    const WorkerLDS ws = worker_lds(smem + ncls_al * 8 + wib * wsb, p.K);
-    char *slice = smem + ncls_al * 8 + wpb * wsb + (int64_t)wib * p.lds_per_worker;
-    for (int i = threadIdx.x; i < p.ncls; i += kThreads) shist[i] = 0;
+    // LDS: single-query class counters of the block | per-worker records |
+    // per-worker class counters (batch) | per-worker stack slices
+    const bool single = p.nq == 1;
+    const int bslots = single && p.qd[0].motif ? (p.qd[0].ncls + 1) & ~1 : 0;
+    const int wslots = single ? 0 : (p.hist_slots + 1) & ~1;
+    const size_t wsb = worker_lds_bytes(p.kmax);
+    unsigned long long *bhist = (unsigned long long *)smem;
+    char *base = smem + (size_t)bslots * 8;
+    const WorkerLDS ws = worker_lds(base + wib * wsb, p.kmax);
+    base += wpb * wsb;
+    unsigned long long *whist = (unsigned long long *)base + (size_t)wib * wslots;
+    base += (size_t)wpb * wslots * 8;
+    char *slice = base + (int64_t)wib * p.lds_per_worker;
+    for (int i = threadIdx.x; i < bslots; i += kThreads) bhist[i] = 0;
+    for (int i = threadIdx.x; i < wpb * wslots; i += kThreads) ((unsigned long long *)(smem + (size_t)bslots * 8 + wpb * wsb))[i] = 0;
+    if (threadIdx.x == 0 && p.start_ticks) atomicMin(p.start_ticks, (unsigned long long)wall_clock64());
     __syncthreads();
 
-    unsigned long long total = 0, bytes = 0, jobs = 0, busy = 0;
+    unsigned long long bytes = 0, jobs = 0, busy = 0, qcount = 0;
     if (wid < p.nworkers) {
-        Worker<W, MOTIF> wk(p, ws, shist, slice, wid);
+        Worker<W, MODE> wk(p, ws, single ? bhist : whist, slice, wid);
         uint32_t seed = 0x9E3779B9u * (uint32_t)(wid + 1);
         bool idle = false, static_done = false, has_ticket = false;
         uint32_t ticket = 0;
@@ -966,7 +1020,7 @@ __global__ __launch_bounds__(kThreads, MOTIF ? 4 : 3) void enum_kernel(EnumParam
                 got = take_static<W>(p, g, wid, ws, jh);
                 if (!got) got = steal_static<W>(p, g, wid, ws, jh, seed, static_done);
             }
-            if (!got && static_done) got = poll_ticket<W>(p, g, ws, jh, has_ticket, ticket);
+            if (!got && static_done) got = poll_ticket<W>(p, g, ws, jh, has_ticket, ticket, wk.epoch);
             if (!got) {
                 if (!idle) {
                     if (g.gl == 0) atomicAdd(&p.sc->idle, 1);
@@ -998,33 +1052,63 @@ __global__ __launch_bounds__(kThreads, MOTIF ? 4 : 3) void enum_kernel(EnumParam
                 if (g.gl == 0) atomicSub(&p.sc->idle, 1);
                 idle = false;
             }
-            const int32_t root = ws.jid[0];
-            if (jh.lvl < 0 || jh.lvl > (p.K == 2 ? 0 : p.K - 3) || root < 0 || root >= p.n) {
+            bool ok = jh.q >= 0 && jh.q < p.nq;
+            if (ok && jh.q != wk.qid) {
+                flush_query<W>(p, g, wk.qid, qcount, whist);
+                qcount = 0;
+                const QueryDesc &Q = p.qd[jh.q];
+                wk.qid = jh.q;
+                wk.K = Q.K;
+                wk.qmotif = Q.motif != 0;
+                wk.lut = Q.lut;
+                ok = (MODE == 2 || (MODE == 1) == wk.qmotif) && Q.K >= 2 && Q.K <= p.kmax;
+            }
+            const int lvl_max = wk.K == 2 ? 0 : wk.K - 3;
+            const int r0 = ws.jid[0];
+            const int r1 = jh.range_end > 0 ? jh.range_end : r0 + 1;
+            if (!ok || jh.lvl < 0 || jh.lvl > lvl_max || r0 < 0 || r1 > p.n || r0 >= r1) {
                 wk.fail(3);
             } else {
-                const uint64_t c0 = __builtin_amdgcn_s_memtime();
-                wk.run_job(jh.lvl, jh.lo, jh.hi, !jh.donated && jh.lvl == 0 && jh.lo == 0);
-                const uint64_t c1 = __builtin_amdgcn_s_memtime();
-                busy += c1 - c0;
-                unsigned long long jc = g.lsum(wk.cnt);
-                if (g.gl == 0) {
-                    total += jc;
-                    if (p.root_cost) atomicAdd(&p.root_cost[root], (unsigned long long)(c1 - c0));
-                    if (p.root_count && jc) atomicAdd(&p.root_count[root], jc);
+                for (int root = r0; root < r1 && !wk.abort; root++) {
+                    if (jh.range_end > 0) {
+                        // range jobs skip roots without a neighbour above them;
+                        // a motif root still counts its own node's bytes
+                        if (p.off[root + 1] - p.ostart[root] <= 0) {
+                            if (wk.qmotif && g.gl == 0)
+                                wk.bytes += 4ull * (unsigned long long)(p.off[root + 1] - p.off[root]);
+                            continue;
+                        }
+                        ws.jid[0] = root;
+                        wsync();
+                    }
+                    const uint64_t c0 = __builtin_amdgcn_s_memtime();
+                    wk.run_job(jh.lvl, jh.lo, jh.hi, !jh.donated && jh.lvl == 0 && jh.lo == 0);
+                    const uint64_t c1 = __builtin_amdgcn_s_memtime();
+                    busy += c1 - c0;
+                    unsigned long long jc = g.lsum(wk.cnt);
+                    if (g.gl == 0) {
+                        qcount += jc;
+                        if (p.root_cost) atomicAdd(&p.root_cost[root], (unsigned long long)(c1 - c0));
+                        if (p.root_count && jc) atomicAdd(&p.root_count[root], jc);
+                    }
                 }
+                if (g.gl == 0 && p.qd[jh.q].done_ticks)
+                    atomicMax(p.qd[jh.q].done_ticks, (unsigned long long)wall_clock64());
             }
             jobs++;
             if (g.gl == 0) atomicSub(&p.sc->outstanding, 1ull);
             if (wk.abort) break;
         }
         if (idle && g.gl == 0) atomicSub(&p.sc->idle, 1);
+        flush_query<W>(p, g, wk.qid, qcount, whist);
         bytes = g.lsum(wk.bytes);
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < p.ncls; i += kThreads)
-        if (shist[i]) atomicAdd(&p.hist[i], shist[i]);
+    if (single && bslots) {
+        for (int i = threadIdx.x; i < p.qd[0].ncls; i += kThreads)
+            if (bhist[i]) atomicAdd(&p.qd[0].out[i], bhist[i]);
+    }
     if (wid < p.nworkers && g.gl == 0) {
-        if (total) atomicAdd(&p.sc->count, total);
         if (bytes) atomicAdd(&p.sc->alg_bytes, bytes);
         if (jobs) atomicAdd(&p.sc->jobs, jobs);
         atomicAdd(&p.sc->busy_sum, busy);
@@ -1038,25 +1122,68 @@ __global__ void fill_u32_kernel(uint32_t *ptr, uint32_t value, size_t count) {
     for (; i < count; i += stride) ptr[i] = value;
 }
 
-// ---------------------------------------------------------------------------
-size_t enum_lds_bytes(int vwave, int threads, int K, int ncls, int lds_per_worker) {
-    int wpb = threads / vwave;
-    size_t ncls_al = (size_t)((ncls + 1) & ~1);
-    return ncls_al * 8 + (size_t)wpb * worker_lds_bytes(K) + (size_t)wpb * lds_per_worker;
+// Resets the scheduler state of a launch; graph-capturable (no memset or
+// host copies).  Bumps the epoch so queue slots of earlier launches never
+// match.
+__global__ void enum_init_kernel(SchedCounters *sc, unsigned long long njobs, int32_t *epoch, int32_t *sq_cur,
+                                 int nworkers, unsigned long long *results, size_t nresults,
+                                 unsigned long long *done_ticks, int nq, unsigned long long *start_ticks) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    if (i == 0) {
+        sc->outstanding = njobs;
+        sc->idle = 0;
+        sc->static_left = (int32_t)(njobs < 0x7fffffffull ? njobs : 0x7fffffffull);
+        sc->q_head = 0;
+        sc->q_tail = 0;
+        sc->error = 0;
+        sc->donations = 0;
+        sc->jobs = 0;
+        sc->alg_bytes = 0;
+        sc->count = 0;
+        sc->busy_sum = 0;
+        sc->busy_max = 0;
+        *epoch = *epoch + 1;
+        if (start_ticks) *start_ticks = ~0ull;
+    }
+    for (size_t t = i; t < (size_t)nworkers; t += stride) sq_cur[t] = 0;
+    for (size_t t = i; t < nresults; t += stride) results[t] = 0;
+    if (done_ticks)
+        for (size_t t = i; t < (size_t)nq; t += stride) done_ticks[t] = 0;
 }
 
-static const void *kernel_for(int vwave, bool motif) {
-    switch (vwave) {
-        case 8: return motif ? (const void *)enum_kernel<8, true> : (const void *)enum_kernel<8, false>;
-        case 16: return motif ? (const void *)enum_kernel<16, true> : (const void *)enum_kernel<16, false>;
-        case 32: return motif ? (const void *)enum_kernel<32, true> : (const void *)enum_kernel<32, false>;
-        case 64: return motif ? (const void *)enum_kernel<64, true> : (const void *)enum_kernel<64, false>;
+// ---------------------------------------------------------------------------
+size_t enum_lds_bytes(int vwave, int threads, int kmax, int ncls_single, int hist_slots, int lds_per_worker) {
+    int wpb = threads / vwave;
+    size_t b = (size_t)((ncls_single + 1) & ~1) * 8;
+    b += (size_t)wpb * worker_lds_bytes(kmax);
+    b += (size_t)wpb * (size_t)((hist_slots + 1) & ~1) * 8;
+    b += (size_t)wpb * lds_per_worker;
+    return b;
+}
+
+template <int W>
+static const void *kernel_mode(int mode) {
+    switch (mode) {
+        case 0: return (const void *)enum_kernel<W, 0>;
+        case 1: return (const void *)enum_kernel<W, 1>;
+        case 2: return (const void *)enum_kernel<W, 2>;
         default: return nullptr;
     }
 }
 
-int enum_occupancy(int vwave, bool motif, int threads, size_t lds, int *blocks_per_cu) {
-    const void *k = kernel_for(vwave, motif);
+static const void *kernel_for(int vwave, int mode) {
+    switch (vwave) {
+        case 8: return kernel_mode<8>(mode);
+        case 16: return kernel_mode<16>(mode);
+        case 32: return kernel_mode<32>(mode);
+        case 64: return kernel_mode<64>(mode);
+        default: return nullptr;
+    }
+}
+
+int enum_occupancy(int vwave, int mode, int threads, size_t lds, int *blocks_per_cu) {
+    const void *k = kernel_for(vwave, mode);
     if (!k || threads != kThreads) return set_error(DM_ERR_ARG, "unsupported launch shape");
     if (lds > 160 * 1024) return set_error(DM_ERR_ARG, "LDS request %zu exceeds 160 KiB", lds);
     if (lds > 64 * 1024) {
@@ -1070,13 +1197,24 @@ int enum_occupancy(int vwave, bool motif, int threads, size_t lds, int *blocks_p
     return DM_OK;
 }
 
-int enum_launch(const EnumParams &p, int vwave, int blocks, int threads, size_t lds, void *stream) {
-    const void *k = kernel_for(vwave, p.motif != 0);
+int enum_launch(const EnumParams &p, int mode, int vwave, int blocks, int threads, size_t lds, void *stream) {
+    const void *k = kernel_for(vwave, mode);
     if (!k || threads != kThreads) return set_error(DM_ERR_ARG, "unsupported launch shape");
     EnumParams copy = p;
     void *args[] = {&copy};
     hipError_t e = hipLaunchKernel(k, dim3(blocks), dim3(threads), args, lds, (hipStream_t)stream);
     if (e != hipSuccess) return set_error(DM_ERR_HIP, "enum launch: %s", hipGetErrorString(e));
+    return DM_OK;
+}
+
+int enum_init_launch(const EnumParams &p, unsigned long long njobs, int32_t *epoch, unsigned long long *results,
+                     size_t nresults, unsigned long long *done_ticks, int nq, void *stream) {
+    size_t work = std::max<size_t>({(size_t)p.nworkers, nresults, (size_t)nq, (size_t)1});
+    int blocks = (int)std::min<size_t>((work + 255) / 256, 1024);
+    hipLaunchKernelGGL(enum_init_kernel, dim3(blocks), dim3(256), 0, (hipStream_t)stream, p.sc, njobs, epoch,
+                       p.sq_cur, p.nworkers, results, nresults, done_ticks, nq, p.start_ticks);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return set_error(DM_ERR_HIP, "init launch: %s", hipGetErrorString(e));
     return DM_OK;
 }
 

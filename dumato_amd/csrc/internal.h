@@ -70,9 +70,9 @@ struct Rng {
 };
 
 // ---- device-side structures (identical layout on host and device) ----------
-// A donated job: children [lo, hi) of the node with prefix ids[0..lvl].
-// Word j holds (launch epoch << 32 | value): words 0..3 = lvl, lo, hi, 0;
-// words 4.. = ids.
+// A donated job: children [lo, hi) of the node with prefix ids[0..lvl] in
+// query q.  Word j holds (launch epoch << 32 | value): words 0..3 = lvl, lo,
+// hi, q; words 4.. = ids.
 constexpr int kJobWords = 4 + kJobIds;
 struct Job {
     uint64_t w[kJobWords];
@@ -102,43 +102,66 @@ struct SchedCounters {
     unsigned long long busy_max;     // largest per-worker busy cycles
 };
 
+// One counting query of a launch.  A single-graph call has one; a batch
+// has one per (graph, pattern, k).
+struct QueryDesc {
+    int32_t K;
+    int32_t motif;
+    int32_t ncls;
+    int32_t graph;                    // graph index in a batch (latency grouping)
+    const uint16_t *lut;              // quick code -> class (motif)
+    unsigned long long *out;          // clique: out[0]; motif: out[0 .. ncls)
+    unsigned long long *done_ticks;   // nullable: wall clock of the query's last job end
+};
+
 struct EnumParams {
-    // graph: symmetric CSR in degree-rank ids; ostart[v] = first index of N(v) above v
+    // graph: symmetric CSR in degree-rank ids (a batch concatenates its graphs,
+    // each relabelled inside its own id range); ostart[v] = first index of
+    // N(v) above v
     const int64_t *off;
     const int32_t *adj;
     const int32_t *ostart;
     int32_t n;
-    int32_t K;
-    int32_t motif;
+    int32_t nq;                       // queries
+    const QueryDesc *qd;
+    int32_t kmax;                     // largest k of the queries (LDS record size)
+    int32_t hist_slots;               // per-worker class counters (batch launches)
     int32_t nworkers;
-    // static partition: worker w owns sq_jobs[sq_off[w] .. sq_off[w+1]) (root, lo, hi)
+    int32_t pad1;
+    // static partition: worker w owns records sq_jobs[4*sq_off[w] .. 4*sq_off[w+1]):
+    //   (root, lo, hi, q)   children [lo, hi) of root (q = query), or
+    //   (r0, -1, r1, q)     every root of [r0, r1), all children
     const int64_t *sq_off;
     const int32_t *sq_jobs;
     int32_t *sq_cur;
     // donated jobs
     Job *q;
     uint32_t q_cap;
-    int32_t epoch;
+    int32_t pad2;
+    const int32_t *epoch;             // launch epoch in device memory (set by the init kernel)
+    unsigned long long *start_ticks;  // wall clock of the first block start
     SchedCounters *sc;
     // per-worker stacks: LDS slice of lds_per_worker bytes, then global arena
     char *arena;
     int64_t arena_per_worker;
     int32_t lds_per_worker;
-    int32_t ncls;
-    // results
-    unsigned long long *hist;
-    const uint16_t *lut;
-    unsigned long long *root_cost;   // nullable, per degree-rank id
-    unsigned long long *root_count;  // nullable
+    int32_t pad3;
+    unsigned long long *root_cost;    // nullable, per vertex
+    unsigned long long *root_count;   // nullable
     int32_t steal, steal_min, steal_period, bitmap;
-    uint64_t watchdog_ticks;         // wall-clock ticks
+    uint64_t watchdog_ticks;          // wall-clock ticks
 };
 
 // kernels (enum_kernel.hip)
-size_t enum_lds_bytes(int vwave, int threads, int K, int ncls, int lds_per_worker);
-int enum_occupancy(int vwave, bool motif, int threads, size_t lds, int *blocks_per_cu);
-int enum_launch(const EnumParams &p, int vwave, int blocks, int threads, size_t lds,
+// mode: 0 cliques only, 1 motifs only, 2 mixed (batch)
+size_t enum_lds_bytes(int vwave, int threads, int kmax, int ncls_single, int hist_slots, int lds_per_worker);
+int enum_occupancy(int vwave, int mode, int threads, size_t lds, int *blocks_per_cu);
+int enum_launch(const EnumParams &p, int mode, int vwave, int blocks, int threads, size_t lds,
                 void *stream);
+// resets the scheduler state of a launch (graph-capturable: kernels only)
+int enum_init_launch(const EnumParams &p, unsigned long long njobs, int32_t *epoch,
+                     unsigned long long *results, size_t nresults, unsigned long long *done_ticks,
+                     int nq, void *stream);
 int fill_u32_launch(void *ptr, uint32_t value, size_t count, void *stream);
 
 }  // namespace dm

@@ -97,7 +97,9 @@ static void device_free(DeviceGraph *d) {
     F(d->d_sq_jobs);
     F(d->d_sq_cur);
     F(d->d_sc);
-    F(d->d_hist);
+    F(d->d_results);
+    F(d->d_qd);
+    F(d->d_epoch);
     F(d->d_root_cost);
     F(d->d_root_count);
     for (auto &l : d->d_lut) F(l);
@@ -110,14 +112,18 @@ static void device_free(DeviceGraph *d) {
 
 template <class T>
 static int grow(T *&ptr, size_t &cap, size_t need) {
-    if (need <= cap && ptr) return DM_OK;
-    if (ptr) (void)hipFree(ptr);
-    ptr = nullptr;
-    cap = 0;
-    size_t alloc = std::max<size_t>(need, 1);
-    hipError_t e = hipMalloc((void **)&ptr, alloc * sizeof(T));
-    if (e != hipSuccess) return set_error(DM_ERR_NOMEM, "hipMalloc(%zu B): %s", alloc * sizeof(T), hipGetErrorString(e));
-    cap = alloc;
+    return dev_grow(ptr, cap, need);
+}
+
+int ensure_queue(DeviceGraph *d) {
+    const size_t qcap = (size_t)1 << 20;
+    if (d->q_cap_alloc >= qcap && d->d_epoch) return DM_OK;
+    int r = grow(d->d_q, d->q_cap_alloc, qcap);
+    if (r) return r;
+    HIPCHK(hipMemset(d->d_q, 0, qcap * sizeof(Job)));  // epoch 0 never matches
+    size_t c = 0;
+    if ((r = grow(d->d_epoch, c, 1))) return r;
+    HIPCHK(hipMemset(d->d_epoch, 0, sizeof(int32_t)));
     return DM_OK;
 }
 
@@ -170,9 +176,9 @@ int pick_shape(dm_graph *g, int K, bool motif, const dm_config &cfg, Shape &sh) 
     lds = (lds + 15) & ~15;
     int ncls = 0;
     if (motif) ncls = pattern_table(K)->ncls;
-    size_t lds_total = enum_lds_bytes(vw, 256, K, ncls, lds);
+    size_t lds_total = enum_lds_bytes(vw, 256, K, ncls, 0, lds);
     int bpc = 0;
-    int r = enum_occupancy(vw, motif, 256, lds_total, &bpc);
+    int r = enum_occupancy(vw, motif ? 1 : 0, 256, lds_total, &bpc);
     if (r) return r;
     if (bpc < 1) return set_error(DM_ERR_ARG, "launch does not fit one CU (LDS %zu B)", lds_total);
     sh.vwave = vw;
@@ -184,18 +190,22 @@ int pick_shape(dm_graph *g, int K, bool motif, const dm_config &cfg, Shape &sh) 
 }
 
 // per-worker global arena bound (see the allocator in enum_kernel.hip)
-static int64_t arena_bound(const dm_graph *g, int K, bool motif) {
-    int64_t md = std::max<int64_t>(g->rcsr.maxdeg, 1);
+int64_t arena_bound_for(int32_t maxdeg, int32_t max_out_, int K, bool motif) {
+    int64_t md = std::max<int64_t>(maxdeg, 1);
     int64_t b = 256;
     if (motif) {
         for (int l = 1; l <= K - 3; l++) b += 10 * (int64_t)(l + 1) * md + 64;
         b += 4 * md + (int64_t)(K - 2) * md + 64;
     } else {
-        int64_t mo = std::max<int64_t>(g->max_out, 1);
+        int64_t mo = std::max<int64_t>(max_out_, 1);
         for (int l = 1; l <= K - 3; l++) b += 4 * mo + 16;
         b += 256 * 4 * 8 + 3 * 257 * 4 + 64;  // local bitmap of a root (out-degree <= 256)
     }
     return (b + 255) & ~(int64_t)255;
+}
+
+static int64_t arena_bound(const dm_graph *g, int K, bool motif) {
+    return arena_bound_for(g->rcsr.maxdeg, g->max_out, K, motif);
 }
 
 // ---------------------------------------------------------------------------
@@ -240,13 +250,14 @@ void plain_partition(const dm_graph *g, const std::vector<int32_t> &roots, int K
         int64_t c = n > w ? (n - w + workers - 1) / workers : 0;
         sq_off[w + 1] = sq_off[w] + c;
     }
-    sq_jobs.assign((size_t)n * 3, 0);
+    sq_jobs.assign((size_t)n * 4, 0);
     for (int w = 0; w < workers; w++) {
         int64_t o = sq_off[w];
         for (int64_t i = w; i < n; i += workers, o++) {
-            sq_jobs[3 * o] = ord[i];
-            sq_jobs[3 * o + 1] = 0;
-            sq_jobs[3 * o + 2] = INT_MAX;
+            sq_jobs[4 * o] = ord[i];
+            sq_jobs[4 * o + 1] = 0;
+            sq_jobs[4 * o + 2] = INT_MAX;
+            sq_jobs[4 * o + 3] = 0;
         }
     }
 }
@@ -297,22 +308,18 @@ int run_enum(dm_graph *g, int K, bool motif, const dm_config &cfg, uint64_t *his
         plain_partition(g, roots, K, motif, sh.workers, sq_off, sq_jobs);
     }
     const double t_s1 = now_ms();
-    const int64_t njobs = (int64_t)sq_jobs.size() / 3;
+    const int64_t njobs = (int64_t)sq_jobs.size() / 4;
 
     // device workspaces
     size_t need_arena = (size_t)arena_bound(g, K, motif) * (size_t)sh.workers;
     if ((r = grow(d->d_arena, d->arena_cap, need_arena))) return r;
     if ((r = grow(d->d_sq_off, d->sq_off_cap, sq_off.size()))) return r;
-    if ((r = grow(d->d_sq_jobs, d->sq_jobs_cap, std::max<size_t>(sq_jobs.size(), 3)))) return r;
+    if ((r = grow(d->d_sq_jobs, d->sq_jobs_cap, std::max<size_t>(sq_jobs.size(), 4)))) return r;
     if ((r = grow(d->d_sq_cur, d->sq_cur_cap, (size_t)sh.workers))) return r;
-    const uint32_t qcap = 1u << 20;
-    if (d->q_cap_alloc < qcap) {
-        if ((r = grow(d->d_q, d->q_cap_alloc, qcap))) return r;
-        HIPCHK(hipMemset(d->d_q, 0, (size_t)qcap * sizeof(Job)));  // epoch 0 never matches
-        d->epoch = 0;
-    }
+    if ((r = ensure_queue(d))) return r;
     const size_t ncls = tab ? (size_t)tab->ncls : 0;
-    if ((r = grow(d->d_hist, d->hist_cap, std::max<size_t>(ncls, 1)))) return r;
+    const size_t nres = motif ? ncls : 1;
+    if ((r = grow(d->d_results, d->results_cap, std::max<size_t>(nres, 1)))) return r;
     if (tab && !d->d_lut[K]) {
         size_t c = 0;
         if ((r = grow(d->d_lut[K], c, tab->q2c.size()))) return r;
@@ -321,18 +328,20 @@ int run_enum(dm_graph *g, int K, bool motif, const dm_config &cfg, uint64_t *his
     const bool want_cost = cfg.record_costs != 0, want_cnt = cfg.per_root != 0;
     if (want_cost && (r = grow(d->d_root_cost, d->root_cost_cap, (size_t)n))) return r;
     if (want_cnt && (r = grow(d->d_root_count, d->root_count_cap, (size_t)n))) return r;
+    QueryDesc qd;
+    memset(&qd, 0, sizeof(qd));
+    qd.K = K;
+    qd.motif = motif ? 1 : 0;
+    qd.ncls = (int32_t)ncls;
+    qd.lut = tab ? d->d_lut[K] : nullptr;
+    qd.out = d->d_results;
+    if ((r = grow(d->d_qd, d->qd_cap, 1))) return r;
 
     hipStream_t s = d->stream;
+    HIPCHK(hipMemcpyAsync(d->d_qd, &qd, sizeof(qd), hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync(d->d_sq_off, sq_off.data(), sq_off.size() * 8, hipMemcpyHostToDevice, s));
     if (!sq_jobs.empty())
         HIPCHK(hipMemcpyAsync(d->d_sq_jobs, sq_jobs.data(), sq_jobs.size() * 4, hipMemcpyHostToDevice, s));
-    SchedCounters sc0;
-    memset(&sc0, 0, sizeof(sc0));
-    sc0.outstanding = (unsigned long long)njobs;
-    sc0.static_left = (int32_t)std::min<int64_t>(njobs, INT_MAX);
-    HIPCHK(hipMemcpyAsync(d->d_sc, &sc0, sizeof(sc0), hipMemcpyHostToDevice, s));
-    if ((r = fill_u32_launch(d->d_sq_cur, 0, (size_t)sh.workers, s))) return r;
-    if (ncls && (r = fill_u32_launch(d->d_hist, 0, ncls * 2, s))) return r;
     if (want_cost && (r = fill_u32_launch(d->d_root_cost, 0, (size_t)n * 2, s))) return r;
     if (want_cnt && (r = fill_u32_launch(d->d_root_count, 0, (size_t)n * 2, s))) return r;
 
@@ -342,22 +351,22 @@ int run_enum(dm_graph *g, int K, bool motif, const dm_config &cfg, uint64_t *his
     p.adj = d->d_adj;
     p.ostart = d->d_ostart;
     p.n = n;
-    p.K = K;
-    p.motif = motif ? 1 : 0;
+    p.nq = 1;
+    p.qd = d->d_qd;
+    p.kmax = K;
+    p.hist_slots = 0;
     p.nworkers = sh.workers;
     p.sq_off = d->d_sq_off;
     p.sq_jobs = d->d_sq_jobs;
     p.sq_cur = d->d_sq_cur;
     p.q = d->d_q;
-    p.q_cap = qcap;
-    p.epoch = ++d->epoch;
+    p.q_cap = (uint32_t)d->q_cap_alloc;
+    p.epoch = d->d_epoch;
+    p.start_ticks = nullptr;
     p.sc = d->d_sc;
     p.arena = (char *)d->d_arena;
     p.arena_per_worker = arena_bound(g, K, motif);
     p.lds_per_worker = sh.lds_per_worker;
-    p.ncls = (int32_t)ncls;
-    p.hist = d->d_hist;
-    p.lut = tab ? d->d_lut[K] : nullptr;
     p.root_cost = want_cost ? d->d_root_cost : nullptr;
     p.root_count = want_cnt ? d->d_root_count : nullptr;
     p.steal = cfg.steal ? 1 : 0;
@@ -365,14 +374,15 @@ int run_enum(dm_graph *g, int K, bool motif, const dm_config &cfg, uint64_t *his
     p.steal_period = cfg.steal_period;
     p.bitmap = cfg.clique_bitmap ? 1 : 0;
     p.watchdog_ticks = cfg.watchdog_s > 0 ? (uint64_t)(cfg.watchdog_s * d->wall_khz * 1000.0) : 0;
+    if ((r = enum_init_launch(p, (unsigned long long)njobs, d->d_epoch, d->d_results, nres, nullptr, 0, s))) return r;
 
     HIPCHK(hipEventRecord(d->ev0, s));
-    if (njobs > 0 && (r = enum_launch(p, sh.vwave, sh.blocks, 256, sh.lds_total, s))) return r;
+    if (njobs > 0 && (r = enum_launch(p, motif ? 1 : 0, sh.vwave, sh.blocks, 256, sh.lds_total, s))) return r;
     HIPCHK(hipEventRecord(d->ev1, s));
     SchedCounters sc1;
     HIPCHK(hipMemcpyAsync(&sc1, d->d_sc, sizeof(sc1), hipMemcpyDeviceToHost, s));
-    std::vector<unsigned long long> h(ncls);
-    if (ncls) HIPCHK(hipMemcpyAsync(h.data(), d->d_hist, ncls * 8, hipMemcpyDeviceToHost, s));
+    std::vector<unsigned long long> h(nres);
+    HIPCHK(hipMemcpyAsync(h.data(), d->d_results, nres * 8, hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
     float kms = 0;
     HIPCHK(hipEventElapsedTime(&kms, d->ev0, d->ev1));
@@ -383,13 +393,10 @@ int run_enum(dm_graph *g, int K, bool motif, const dm_config &cfg, uint64_t *his
     uint64_t total = 0;
     if (motif) {
         for (size_t i = 0; i < ncls; i++) total += h[i];
-        if (total != sc1.count)
-            return set_error(DM_ERR_DEVICE, "histogram sum %llu != leaf count %llu", (unsigned long long)total,
-                             (unsigned long long)sc1.count);
         if (hist)
             for (size_t i = 0; i < ncls; i++) hist[i] = h[i];
     } else {
-        total = sc1.count;
+        total = h[0];
     }
     if (count_out) *count_out = total;
 

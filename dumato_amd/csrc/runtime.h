@@ -32,8 +32,11 @@ struct DeviceGraph {
     int32_t *d_sq_cur = nullptr;
     size_t sq_cur_cap = 0;
     SchedCounters *d_sc = nullptr;
-    unsigned long long *d_hist = nullptr;
-    size_t hist_cap = 0;
+    unsigned long long *d_results = nullptr;  // clique count or motif histogram
+    size_t results_cap = 0;
+    QueryDesc *d_qd = nullptr;
+    size_t qd_cap = 0;
+    int32_t *d_epoch = nullptr;
     unsigned long long *d_root_cost = nullptr;
     size_t root_cost_cap = 0;
     unsigned long long *d_root_count = nullptr;
@@ -41,7 +44,6 @@ struct DeviceGraph {
     uint16_t *d_lut[kMaxMotifK + 1] = {nullptr};
     float *d_mlp = nullptr;  // MLP workspace (sched.cpp)
     size_t mlp_cap = 0;
-    int32_t epoch = 0;
 };
 
 struct Shape {
@@ -72,8 +74,24 @@ struct dm_graph {
 };
 
 namespace dm {
+// grow-only device buffer
+template <class T>
+inline int dev_grow(T *&ptr, size_t &cap, size_t need) {
+    if (need <= cap && ptr) return DM_OK;
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    cap = 0;
+    size_t alloc = need > 0 ? need : 1;
+    hipError_t e = hipMalloc((void **)&ptr, alloc * sizeof(T));
+    if (e != hipSuccess)
+        return set_error(DM_ERR_NOMEM, "hipMalloc(%zu B): %s", alloc * sizeof(T), hipGetErrorString(e));
+    cap = alloc;
+    return DM_OK;
+}
+int64_t arena_bound_for(int32_t maxdeg, int32_t max_out, int K, bool motif);
 int graph_finish(dm_graph *g);
 int ensure_device(dm_graph *g);
+int ensure_queue(DeviceGraph *d);
 int pick_shape(dm_graph *g, int K, bool motif, const dm_config &cfg, Shape &sh);
 bool root_has_work(const dm_graph *g, int32_t v, int K, bool motif);
 void plain_partition(const dm_graph *g, const std::vector<int32_t> &roots, int K, bool motif, int workers,

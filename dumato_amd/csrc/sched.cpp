@@ -453,13 +453,14 @@ int learned_partition(dm_graph *g, const std::vector<int32_t> &roots, int K, boo
     }
     sq_off.assign((size_t)workers + 1, 0);
     for (int w = 0; w < workers; w++) sq_off[w + 1] = sq_off[w] + (int64_t)q[w].size();
-    sq_jobs.assign((size_t)sq_off[workers] * 3, 0);
+    sq_jobs.assign((size_t)sq_off[workers] * 4, 0);
     for (int w = 0; w < workers; w++) {
         int64_t o = sq_off[w];
         for (int32_t pi : q[w]) {
-            sq_jobs[3 * o] = pieces[pi].root;
-            sq_jobs[3 * o + 1] = pieces[pi].lo;
-            sq_jobs[3 * o + 2] = pieces[pi].hi;
+            sq_jobs[4 * o] = pieces[pi].root;
+            sq_jobs[4 * o + 1] = pieces[pi].lo;
+            sq_jobs[4 * o + 2] = pieces[pi].hi;
+            sq_jobs[4 * o + 3] = 0;
             o++;
         }
     }

@@ -7,7 +7,7 @@ Seeds are fixed here: BASE_SEED + workload number.
   4  Barabasi-Albert n=100000, m=10, + 200 planted cliques of size U[8,12]
                                                           motif k=5
   5  256 graphs, n log-uniform [500, 50000], average degree U[4, 40], half
-     Erdos-Renyi, half Chung-Lu (gamma U[2.1, 2.8], wmax = min(n/4, 25*avg
+     Erdos-Renyi, half Chung-Lu (gamma U[2.1, 2.8], wmax = min(n/8, 4*avg
      degree)); per graph clique k=3..6 and motif k=3..5
 """
 import math
@@ -47,7 +47,7 @@ def batch_specs(count=256, seed=BASE_SEED + 5):
             specs.append(dict(kind="er", n=n, m=m, seed=int(seed * 1000 + i)))
         else:
             gamma = float(rng.uniform(2.1, 2.8))
-            wmax = float(min(n / 4.0, 25.0 * avg))
+            wmax = float(min(n / 8.0, 4.0 * avg))
             specs.append(dict(kind="chunglu", n=n, m=m, gamma=gamma, wmax=wmax, seed=int(seed * 1000 + i)))
     return specs
 

@@ -155,6 +155,38 @@ int dm_mlp_grad(const float *params, const float *X, const float *Y, int n,
 int dm_mlp_adam(float *params, const float *X, const float *Y, int n, int steps,
                 float lr, float *losses);
 
+/* ---- batches (no reference counterpart; the reference runs one graph and
+ * one k per executable, src/main/clique_PA.cu:34-130) ---- */
+typedef struct dm_batch dm_batch;
+typedef struct dm_query {
+    int32_t graph;    /* index into the batch's graphs */
+    int32_t motif;    /* 0: k-clique count, 1: k-motif histogram (k <= 6) */
+    int32_t k;
+    int32_t reserved;
+} dm_query;
+/* Copies the graphs (their degree-rank CSRs) into one batch; the graphs may
+ * be freed afterwards. */
+int dm_batch_create(dm_graph *const *graphs, size_t ngraphs, dm_batch **out);
+/* Result slots the queries need: 1 per clique query, dm_motif_num_classes(k)
+ * per motif query. */
+int dm_batch_num_results(const dm_batch *b, const dm_query *queries, size_t nq,
+                         size_t *nres);
+/* Runs every query in one persistent kernel launch, captured in a hipGraph
+ * the first time a query list / launch shape is seen and replayed after.
+ * results[result_offsets[i] ..) holds query i's count or histogram;
+ * graph_latency_ms[g] (nullable) = time from launch start until graph g's
+ * last query finished. */
+int dm_batch_run(dm_batch *b, const dm_query *queries, size_t nq,
+                 const dm_config *cfg, uint64_t *results, size_t results_len,
+                 uint64_t *result_offsets, double *graph_latency_ms);
+/* Times whole-batch replays for widths 64/32/16 and steal granularities 2/8
+ * within about budget_ms and keeps the fastest for later dm_batch_run calls
+ * with vwave = 0. */
+int dm_batch_autotune(dm_batch *b, const dm_query *queries, size_t nq,
+                      double budget_ms, dm_config *best);
+int dm_batch_last_stats(const dm_batch *b, dm_stats *st);
+void dm_batch_free(dm_batch *b);
+
 const char *dm_last_error(void);
 int dm_version(void);
 
