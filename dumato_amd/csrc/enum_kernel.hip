@@ -417,66 +417,100 @@ struct Worker {
     }
 
     // children [lo, hi) of level l = K-3, each a node of level K-2 whose
-    // leaves are counted by class without being materialised.
+    // leaves are counted by class without being materialised.  Children are
+    // taken in descending order, in batches of up to W; the batch's
+    // (child, neighbour) pairs are flattened across the lanes so short
+    // neighbour lists do not leave lanes idle.
     DEV void motif_leaf_phase(int l, int lo, int hi) {
         Lvl &P = ws.lv[l];
-        const int npm = 1 << (l + 1);
-        const int top = 1 << (l + 1);
+        const int npm = 1 << (l + 1);  // mask values over the lists 0..l
+        const int top = npm;           // bit of the child's own list
         const int s = quick_off(l + 2);
         const int o1 = quick_off(l + 1);
-        for (int j = g.gl; j < npm; j += W) {
-            ws.n_gt[j] = 0;
-            ws.n_int[j] = 0;
-        }
+        char *tl = lds_top, *tg = g_top;
+        uint32_t *ic = (uint32_t *)alloc((int64_t)W * npm * 4);  // [slot][mask] hits above the child
+        uint32_t *nn = (uint32_t *)alloc((int64_t)W * 4);        // [slot] neighbours not in U
+        int32_t *pre = (int32_t *)alloc((int64_t)(W + 1) * 4);
+        int32_t *cc = (int32_t *)alloc((int64_t)W * 4);          // [slot] child vertex
+        int64_t *cs = (int64_t *)alloc((int64_t)W * 8);          // [slot] adj index of N(c) above v0
+        if (abort) return;
+        // ws.n_gt[m]: valid elements of E above the current batch, by mask
+        for (int j = g.gl; j < npm; j += W) ws.n_gt[j] = 0;
+        for (int j = g.gl; j < W * npm; j += W) ic[j] = 0;
+        for (int j = g.gl; j < W; j += W) nn[j] = 0;
         wsync();
-        int gt_local = 0;
-        for (int j = hi + g.gl; j < P.ne; j += W) {
-            atomicAdd(&ws.n_gt[epm_at(P, j)], 1u);
-            gt_local++;
-        }
-        int gt_total = g.isum(gt_local);
+        for (int j = hi + g.gl; j < P.ne; j += W) atomicAdd(&ws.n_gt[epm_at(P, j)], 1u);
         wsync();
         const int pnu = P.nu;
-        for (int i = hi - 1; i >= lo; i--) {
+        const int target = 2 * W;
+        int i = hi - 1;
+        while (i >= lo) {
             if (check_abort()) return;
             maybe_donate(l, l, lo, i);
             if (i < lo) break;
-            const int32_t c = P.eid[i];
-            const int pmc = epm_at(P, i);
-            const uint64_t indc = P.ind + ((l + 1 >= 2) ? ((uint64_t)pmc << o1) : 0);
-            const int32_t *nc = nbr(c);
-            const int dc = deg(c);
-            const int s0 = lower_bound(nc, dc, v0 + 1);
-            int my_new = 0;
-            for (int t = s0 + g.gl; t < dc; t += W) {
-                int32_t x = nc[t];
-                int pos = lower_bound(P.uid, pnu, x);
+            const int avail = i - lo + 1 < W ? i - lo + 1 : W;
+            int len = 0, pm_a = -1;
+            if (g.gl < avail) {
+                const int32_t c = P.eid[i - g.gl];
+                const int dc = deg(c);
+                const int s0 = lower_bound(nbr(c), dc, v0 + 1);
+                len = dc - s0;
+                cc[g.gl] = c;
+                cs[g.gl] = p.off[c] + s0;
+                pm_a = epm_at(P, i - g.gl);
+            }
+            const int sc = g.iscan(len);
+            int take = __popcll(g.ballot(g.gl < avail && sc - len < target));
+            take = take < 1 ? 1 : take;
+            if (g.gl < take) pre[g.gl + 1] = sc;
+            if (g.gl == 0) pre[0] = 0;
+            const int tot = g.bcast(sc, take - 1);
+            wsync();
+            int a = 0;
+            for (int t = g.gl; t < tot; t += W) {
+                while (pre[a + 1] <= t) a++;
+                const int32_t x = p.adj[cs[a] + (t - pre[a])];
+                const int pos = lower_bound(P.uid, pnu, x);
                 if (pos < pnu && P.uid[pos] == x) {
-                    uint8_t pm = upm_at(P, pos);
-                    if ((pm & 0x80) && x > c) atomicAdd(&ws.n_int[pm & 0x7f], 1u);
+                    const uint8_t pm = upm_at(P, pos);
+                    if ((pm & 0x80) && x > cc[a]) atomicAdd(&ic[a * npm + (pm & 0x7f)], 1u);
                 } else {
-                    my_new++;
+                    atomicAdd(&nn[a], 1u);
                 }
             }
-            const int n_new = g.isum(my_new);
             wsync();
-            for (int j = g.gl; j < npm; j += W) {
-                uint32_t a = ws.n_int[j];
-                uint32_t b = ws.n_gt[j] - a;
-                if (a) atomicAdd(&shist[lut[indc + ((uint64_t)(j | top) << s)]], (unsigned long long)a);
-                if (b) atomicAdd(&shist[lut[indc + ((uint64_t)j << s)]], (unsigned long long)b);
-                ws.n_int[j] = 0;
+            // per child: slot a = lane a; valid elements above it = the
+            // batch base plus earlier slots (larger E indices) with that mask
+            const bool mine = g.gl < take;
+            const uint64_t indc = P.ind + ((mine && l + 1 >= 2) ? ((uint64_t)pm_a << o1) : 0);
+            unsigned long long leaves = 0;
+            for (int j = 0; j < npm; j++) {
+                const uint64_t mj = g.ballot(mine && pm_a == j);
+                if (mine) {
+                    const uint32_t gtj = ws.n_gt[j] + (uint32_t)g.below(mj);
+                    const uint32_t ai = ic[g.gl * npm + j];
+                    const uint32_t bi = gtj - ai;
+                    if (ai) atomicAdd(&shist[lut[indc + ((uint64_t)(j | top) << s)]], (unsigned long long)ai);
+                    if (bi) atomicAdd(&shist[lut[indc + ((uint64_t)j << s)]], (unsigned long long)bi);
+                    ic[g.gl * npm + j] = 0;
+                    leaves += gtj;
+                }
+                wsync();
+                if (g.gl == 0) ws.n_gt[j] += (uint32_t)__popcll(mj);
             }
-            if (g.gl == 0) {
-                if (n_new) atomicAdd(&shist[lut[indc + ((uint64_t)top << s)]], (unsigned long long)n_new);
-                ws.n_gt[pmc] += 1;
-                const unsigned long long leaves = (unsigned long long)n_new + (unsigned long long)gt_total;
+            if (mine) {
+                const uint32_t nw = nn[g.gl];
+                if (nw) atomicAdd(&shist[lut[indc + ((uint64_t)top << s)]], (unsigned long long)nw);
+                nn[g.gl] = 0;
+                leaves += nw;
                 cnt += leaves;
-                bytes += 4ull * (unsigned long long)(P.sdeg + dc) + 4ull * (K - 1) * leaves;
+                bytes += 4ull * (unsigned long long)(P.sdeg + deg(cc[g.gl])) + 4ull * (K - 1) * leaves;
             }
-            gt_total++;
             wsync();
+            i -= take;
         }
+        lds_top = tl;
+        g_top = tg;
     }
 
     // ---- clique levels ----------------------------------------------------------
@@ -981,7 +1015,7 @@ DEV void flush_query(const EnumParams &p, const Grp<W> &g, int q, unsigned long 
 
 // MODE 0: cliques only, 1: motifs only, 2: mixed (batch launches).
 template <int W, int MODE>
-__global__ __launch_bounds__(kThreads, MODE == 1 ? 4 : 3) void enum_kernel(EnumParams p) {
+__global__ __launch_bounds__(kThreads, MODE == 1 ? 3 : 3) void enum_kernel(EnumParams p) {
     extern __shared__ __align__(16) char smem[];
     Grp<W> g;
     const int wpb = kThreads / W;
