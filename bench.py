@@ -40,6 +40,14 @@ HBM_PEAK_GBS = 8000.0
 PROFILE_TRAFFIC = os.path.join(ROOT, "profiles", "traffic.json")
 
 
+def progress(msg):
+    """One flushed stderr line per finished stage (the JSON line stays the only stdout)."""
+    print("[bench %.1fs] %s" % (time.perf_counter() - T0, msg), file=sys.stderr, flush=True)
+
+
+T0 = time.perf_counter()
+
+
 def cpu_baseline(graph, mode, k, budget_s, seed=7):
     """Oracle port timed on random roots until budget_s is used."""
     import numpy as np
@@ -66,6 +74,7 @@ def cpu_baseline(graph, mode, k, budget_s, seed=7):
         per_root = el / done
         left = budget_s - el
         chunk = int(max(16, min(4096, 0.9 * left / per_root if per_root > 0 else 4096)))
+        progress("cpu baseline: %d roots, %.1f s" % (done, el))
     dt = time.perf_counter() - t0
     return {"value": count / dt if dt > 0 else 0.0, "unit": "subgraphs/s", "cores": threads,
             "kind": "port",
@@ -108,10 +117,14 @@ def plain_vs_learned(graph, mode, k_list, k_train, warmup, steps, budget_ms=300.
     t = time.perf_counter()
     loss = graph.sched_train(mode == "motif", k_train, epochs=30, seed=1)
     out["train"] = {"k_train": k_train, "loss": loss, "seconds": time.perf_counter() - t}
+    progress("%s: cost model trained on k=%d (loss %.3f)" % (mode, k_train, loss))
     for k in k_list:
         graph.autotune(mode == "motif", k, budget_ms)
+        progress("%s k=%d: autotuned" % (mode, k))
         p = query_block(graph, mode, k, dm.DM_SCHED_PLAIN, warmup, steps)
+        progress("%s k=%d plain: kernel %.2f ms" % (mode, k, p["kernel_ms"]))
         lq = query_block(graph, mode, k, dm.DM_SCHED_LEARNED, warmup, steps)
+        progress("%s k=%d learned: kernel %.2f ms" % (mode, k, lq["kernel_ms"]))
         assert p["subgraphs"] == lq["subgraphs"], "learned and plain counts differ"
         out["k%d" % k] = {"plain": p, "learned": lq,
                           "learned_speedup": p["query_ms"] / lq["query_ms"],
@@ -131,7 +144,9 @@ def batch_block(warmup, steps, check_graphs=8):
     queries = [(gi, mode, k) for gi in range(len(graphs)) for (mode, k) in W.BATCH_QUERIES]
     b = dm.Batch(graphs)
     gen_s = time.perf_counter() - t
-    best = b.autotune(queries, budget_ms=3000.0)
+    progress("batch: %d graphs built in %.1f s" % (len(graphs), gen_s))
+    best = b.autotune(queries, budget_ms=1500.0)
+    progress("batch: autotuned (vwave %d)" % best.vwave)
     for _ in range(warmup):
         b.run(queries)
     kms, lats = [], []
@@ -142,6 +157,7 @@ def batch_block(warmup, steps, check_graphs=8):
         kms.append(b.last_stats()["device_ms"])
         lats.append(lat)
     wall = time.perf_counter() - t0
+    progress("batch: %d replays timed" % steps)
     total = sum(int(np.asarray(r).sum()) for r in res)
     # spot-check: the first graphs' results against the single-graph path
     ok = True
@@ -170,6 +186,8 @@ def main():
     ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--full", action="store_true", help="also the k=5 motif run on the BA graph (~1 min)")
     ap.add_argument("--no-batch", action="store_true", help="skip the 256-graph batched launch")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="headline with the default launch shape (no tuning dispatches; for profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -182,20 +200,24 @@ def main():
 
     g = W.chunglu_18k()
     K = 4
-    g.autotune(True, K, 300.0)
+    if not args.no_autotune:
+        g.autotune(True, K, 300.0)
+    progress("headline graph built%s" % ("" if args.no_autotune else " and autotuned"))
     cfg = dm.config()
 
     def step():
         return g.motif_count(K, cfg)[0]
 
     total, elapsed, kms, _ = timed(step, g, args.warmup, args.steps)
+    progress("headline: %d steps timed, kernel %.2f ms" % (args.steps, statistics.median(kms)))
     st = g.last_stats()
     value = args.steps * total / elapsed
     kern_ms = statistics.median(kms)
     achieved = st["alg_bytes"] / (kern_ms / 1e3) / 1e9
     traffic = None
     if os.path.exists(PROFILE_TRAFFIC):
-        traffic = json.load(open(PROFILE_TRAFFIC)).get("motif4_chunglu18k")
+        rec = json.load(open(PROFILE_TRAFFIC)).get("motif4_chunglu18k")
+        traffic = rec["bytes"] if rec else None
     line = {
         "metric": "subgraphs enumerated/sec (k=4 motif on Chung-Lu 18K; k=5 clique and learned speedup below)",
         "value": value,

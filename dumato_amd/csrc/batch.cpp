@@ -31,7 +31,7 @@ namespace dm {
 
 struct BatchPlan {
     std::vector<dm_query> queries;
-    int vwave = 0, lds = 0, steal = 1, steal_min = 2, steal_period = 8, bitmap = 1, blocks = 0;
+    int vwave = 0, lds = 0, steal = 1, steal_min = 2, steal_period = 2, bitmap = 1, blocks = 0;
     double watchdog_s = 0;
     bool same(const std::vector<dm_query> &q, const dm_config &c, int vw, int l, int sm) const {
         if (q.size() != queries.size()) return false;
@@ -245,7 +245,7 @@ static int batch_build(dm_batch *b, const std::vector<dm_query> &qs, const dm_co
     p.lds_per_worker = lds;
     p.steal = cfg.steal ? 1 : 0;
     p.steal_min = sm;
-    p.steal_period = cfg.steal_period > 0 ? cfg.steal_period : 8;
+    p.poll_ticks = (uint64_t)(cfg.steal_period >= 0 ? cfg.steal_period : 2) * (uint64_t)d->wall_khz / 1000;
     p.bitmap = cfg.clique_bitmap ? 1 : 0;
     p.watchdog_ticks = cfg.watchdog_s > 0 ? (uint64_t)(cfg.watchdog_s * d->wall_khz * 1000.0) : 0;
     // capture: init kernel + persistent kernel

@@ -43,6 +43,9 @@ namespace dm {
 #define DEV __device__ __forceinline__
 
 constexpr int kThreads = 256;
+// Every instantiation is compiled for at least 3 waves per SIMD (at most 168
+// VGPRs); the full-wave motif kernel fits 128 and so reaches 4.
+constexpr int kMinWaves = 3;
 constexpr int kMaxPm = 32;  // 2^(K-2) mask values for K <= 7
 
 // ---------------------------------------------------------------------------
@@ -72,7 +75,12 @@ struct Grp {
         for (int o = W / 2; o > 0; o >>= 1) x += __shfl_xor(x, o, W);
         return x;
     }
-    DEV int bcast(int x, int src) const { return __shfl(x, src, W); }
+    // src is the same for every lane of the worker at every call site; for a
+    // full wave the value is then read into a scalar register
+    DEV int bcast(int x, int src) const {
+        if constexpr (W == 64) return __builtin_amdgcn_readlane(x, src);
+        else return __shfl(x, src, W);
+    }
     DEV int iscan(int x) const {  // inclusive prefix sum over the worker's lanes
 #pragma unroll
         for (int o = 1; o < W; o <<= 1) {
@@ -82,7 +90,13 @@ struct Grp {
         return x;
     }
     DEV unsigned long long bcast64(unsigned long long x, int src) const {
-        return __shfl(x, src, W);
+        if constexpr (W == 64) {
+            const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)x, src);
+            const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(x >> 32), src);
+            return ((unsigned long long)hi << 32) | lo;
+        } else {
+            return __shfl(x, src, W);
+        }
     }
 };
 
@@ -168,7 +182,16 @@ struct Worker {
     unsigned long long bytes;  // per-lane partial algorithmic bytes
     unsigned long long donated;
     uint64_t t_start;
+    uint64_t last_poll;  // wall clock of the last donation check
     bool abort;
+    // motif leaf-phase scratch, reserved first in the LDS slice of each job:
+    // cs int64[B] | ic u32[B][npm] | nn u32[B] | cc i32[B] | pre i32[B + 1]
+    char *lf;
+    static constexpr int kLeafSlots = W < 32 ? W : 32;
+    DEV static int64_t leaf_scratch_bytes(int K) {
+        const int B = kLeafSlots;
+        return (int64_t)B * 8 + (int64_t)B * (1 << (K - 2)) * 4 + (int64_t)B * 12 + 4;
+    }
 
     DEV Worker(const EnumParams &p_, WorkerLDS ws_, unsigned long long *sh_, char *lds_slice,
                int wid)
@@ -186,6 +209,7 @@ struct Worker {
         cnt = bytes = donated = 0;
         abort = false;
         t_start = now_ticks();
+        last_poll = t_start;
     }
 
     // ---- helpers -----------------------------------------------------------
@@ -236,12 +260,12 @@ struct Worker {
         int ok = 0;
         uint32_t slot = 0;
         if (g.gl == 0) {
-            uint32_t t = __hip_atomic_load(&p.sc->q_tail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            uint32_t t = (uint32_t)__hip_atomic_load(&p.sc->q_ht, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             if (t < p.q_cap) {
                 // count the job before it becomes visible, so outstanding
                 // cannot reach zero while it waits in the queue
                 atomicAdd(&p.sc->outstanding, 1ull);
-                t = atomicAdd(&p.sc->q_tail, 1u);
+                t = (uint32_t)atomicAdd(&p.sc->q_ht, 1ull);
                 if (t < p.q_cap) {
                     ok = 1;
                     slot = t;
@@ -271,18 +295,39 @@ struct Worker {
     // Donate the lowest level's remaining children when workers are idle.
     // leaf_k / leaf_lo / leaf_i describe the descending leaf-phase loop of
     // level leaf_k (remaining children [lo, i]); leaf_k < 0 when not in it.
+    // Child index splitting [cur, end) into two halves of equal estimated
+    // work.  A clique child i of a list of n candidates roots about
+    // (n - i)^e subtrees below it, e = K - 3 - l; motif children and the leaf
+    // level are split in half.
+    DEV int split_point(int l, int cur, int end) const {
+        const int half = cur + ((end - cur) >> 1);
+        const bool clique = MODE == 0 || (MODE == 2 && !qmotif);
+        const int e = K - 3 - l;
+        if (!clique || e < 1) return half;
+        const float n = (float)ws.lv[l].nu;
+        const float x = 1.0f / (float)(e + 1);
+        const float a = powf(n - (float)cur, (float)(e + 1));
+        const float b = powf(n - (float)end, (float)(e + 1));
+        int m = (int)(n - powf(0.5f * (a + b), x));
+        if (m <= cur) m = cur + 1;
+        if (m >= end) m = end - 1;
+        return m;
+    }
+
     DEV void maybe_donate(int top, int leaf_k, int &leaf_lo, int leaf_i) {
-        if (!p.steal || (steps % p.steal_period) != 0) return;
-        // donate only when some worker waits, the queue cannot feed it and
-        // no static job is left to steal
+        if (!p.steal) return;
+        const uint64_t now = now_ticks();
+        if (now - last_poll < p.poll_ticks) return;
+        last_poll = now;
+        // donate only when tickets wait for jobs (a worker takes a ticket once
+        // no static job is left).  The check is a hint: a workgroup-scope
+        // load may be served from this CU's cache and lag by a moment, which
+        // only delays a donation; the push itself is exact.
         int want = 0;
         if (g.gl == 0) {
-            int idle = __hip_atomic_load(&p.sc->idle, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            int left = __hip_atomic_load(&p.sc->static_left, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            uint32_t h = __hip_atomic_load(&p.sc->q_head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            uint32_t t = __hip_atomic_load(&p.sc->q_tail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            int queued = (t > h) ? (int)(t - h) : 0;  // h > t: tickets waiting for jobs
-            want = left <= 0 && idle > queued;
+            const unsigned long long ht =
+                __hip_atomic_load(&p.sc->q_ht, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            want = (uint32_t)(ht >> 32) > (uint32_t)ht;
         }
         if (!g.bcast(want, 0)) return;
         // the last level's children are cheap: split it only in bigger pieces
@@ -290,7 +335,7 @@ struct Worker {
         for (int l = job_lvl; l < top; l++) {
             int cur = ws.lv[l].cur, end = ws.lv[l].end;
             if (end - cur >= (l == K - 3 ? leaf_min : p.steal_min)) {
-                int mid = cur + ((end - cur) >> 1);
+                const int mid = split_point(l, cur, end);
                 if (push_job(l, mid, end)) {
                     ws.lv[l].end = mid;
                     donated++;
@@ -427,17 +472,16 @@ struct Worker {
         const int top = npm;           // bit of the child's own list
         const int s = quick_off(l + 2);
         const int o1 = quick_off(l + 1);
-        char *tl = lds_top, *tg = g_top;
-        uint32_t *ic = (uint32_t *)alloc((int64_t)W * npm * 4);  // [slot][mask] hits above the child
-        uint32_t *nn = (uint32_t *)alloc((int64_t)W * 4);        // [slot] neighbours not in U
-        int32_t *pre = (int32_t *)alloc((int64_t)(W + 1) * 4);
-        int32_t *cc = (int32_t *)alloc((int64_t)W * 4);          // [slot] child vertex
-        int64_t *cs = (int64_t *)alloc((int64_t)W * 8);          // [slot] adj index of N(c) above v0
-        if (abort) return;
+        constexpr int B = kLeafSlots;
+        int64_t *cs = (int64_t *)lf;
+        uint32_t *ic = (uint32_t *)(cs + B);
+        uint32_t *nn = ic + B * npm;
+        int32_t *cc = (int32_t *)(nn + B);
+        int32_t *pre = cc + B;
         // ws.n_gt[m]: valid elements of E above the current batch, by mask
         for (int j = g.gl; j < npm; j += W) ws.n_gt[j] = 0;
-        for (int j = g.gl; j < W * npm; j += W) ic[j] = 0;
-        for (int j = g.gl; j < W; j += W) nn[j] = 0;
+        for (int j = g.gl; j < B * npm; j += W) ic[j] = 0;
+        for (int j = g.gl; j < B; j += W) nn[j] = 0;
         wsync();
         for (int j = hi + g.gl; j < P.ne; j += W) atomicAdd(&ws.n_gt[epm_at(P, j)], 1u);
         wsync();
@@ -448,7 +492,7 @@ struct Worker {
             if (check_abort()) return;
             maybe_donate(l, l, lo, i);
             if (i < lo) break;
-            const int avail = i - lo + 1 < W ? i - lo + 1 : W;
+            const int avail = i - lo + 1 < B ? i - lo + 1 : B;
             int len = 0, pm_a = -1;
             if (g.gl < avail) {
                 const int32_t c = P.eid[i - g.gl];
@@ -509,8 +553,6 @@ struct Worker {
             wsync();
             i -= take;
         }
-        lds_top = tl;
-        g_top = tg;
     }
 
     // ---- clique levels ----------------------------------------------------------
@@ -788,6 +830,10 @@ struct Worker {
         g_top = g_lo;
         cnt = 0;
         const bool motif = MODE == 2 ? qmotif : MODE == 1;
+        if (motif) {
+            lf = alloc(leaf_scratch_bytes(K));
+            if (abort) return;
+        }
         if (!motif && K == 2) {
             if (g.gl == 0) {
                 int d = odeg(v0);
@@ -931,7 +977,7 @@ DEV bool take_static(const EnumParams &p, const Grp<W> &g, int w, const WorkerLD
 }
 
 // Donated jobs are served in ticket order: an idle worker takes one ticket
-// (atomicAdd on q_head) and from then on polls only the slot of its ticket,
+// (atomicAdd on the high half of q_ht) and from then on polls only the slot of its ticket,
 // so waiting workers never contend on a shared word.  A slot is complete when
 // all its words carry this launch's epoch.  Returns true when the ticket's job
 // has arrived (and clears the ticket).
@@ -940,7 +986,7 @@ DEV bool poll_ticket(const EnumParams &p, const Grp<W> &g, const WorkerLDS &ws, 
                      uint32_t &ticket, uint32_t ep) {
     if (!has_ticket) {
         uint32_t t = 0;
-        if (g.gl == 0) t = atomicAdd(&p.sc->q_head, 1u);
+        if (g.gl == 0) t = (uint32_t)(atomicAdd(&p.sc->q_ht, 1ull << 32) >> 32);
         ticket = (uint32_t)g.bcast((int)t, 0);
         has_ticket = true;
     }
@@ -1015,11 +1061,11 @@ DEV void flush_query(const EnumParams &p, const Grp<W> &g, int q, unsigned long 
 
 // MODE 0: cliques only, 1: motifs only, 2: mixed (batch launches).
 template <int W, int MODE>
-__global__ __launch_bounds__(kThreads, MODE == 1 ? 3 : 3) void enum_kernel(EnumParams p) {
+__global__ __launch_bounds__(kThreads, kMinWaves) void enum_kernel(EnumParams p) {
     extern __shared__ __align__(16) char smem[];
     Grp<W> g;
     const int wpb = kThreads / W;
-    const int wib = threadIdx.x / W;
+    const int wib = uni<W>(threadIdx.x / W);
     const int wid = blockIdx.x * wpb + wib;
     // LDS: single-query class counters of the block | per-worker records |
     // per-worker class counters (batch) | per-worker stack slices
@@ -1043,9 +1089,10 @@ __global__ __launch_bounds__(kThreads, MODE == 1 ? 3 : 3) void enum_kernel(EnumP
     if (wid < p.nworkers) {
         Worker<W, MODE> wk(p, ws, single ? bhist : whist, slice, wid);
         uint32_t seed = 0x9E3779B9u * (uint32_t)(wid + 1);
-        bool idle = false, static_done = false, has_ticket = false;
+        bool static_done = false, has_ticket = false;
         uint32_t ticket = 0;
         int backoff = 1;
+        uint32_t spins = 0;
         const uint64_t t0 = wk.t_start;
         while (true) {
             JobHead jh;
@@ -1056,19 +1103,19 @@ __global__ __launch_bounds__(kThreads, MODE == 1 ? 3 : 3) void enum_kernel(EnumP
             }
             if (!got && static_done) got = poll_ticket<W>(p, g, ws, jh, has_ticket, ticket, wk.epoch);
             if (!got) {
-                if (!idle) {
-                    if (g.gl == 0) atomicAdd(&p.sc->idle, 1);
-                    idle = true;
+                // the termination words are shared by every worker: read
+                // them on every 4th empty poll only
+                if ((++spins & 3) == 0) {
+                    unsigned long long out = 0;
+                    int err = 0;
+                    if (g.gl == 0) {
+                        out = __hip_atomic_load(&p.sc->outstanding, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        err = __hip_atomic_load(&p.sc->error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    }
+                    out = g.bcast64(out, 0);
+                    err = g.bcast(err, 0);
+                    if (out == 0 || err) break;
                 }
-                unsigned long long out = 0;
-                int err = 0;
-                if (g.gl == 0) {
-                    out = __hip_atomic_load(&p.sc->outstanding, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    err = __hip_atomic_load(&p.sc->error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                }
-                out = g.bcast64(out, 0);
-                err = g.bcast(err, 0);
-                if (out == 0 || err) break;
                 if (p.watchdog_ticks && now_ticks() - t0 > p.watchdog_ticks) {
                     if (g.gl == 0) atomicCAS(&p.sc->error, 0, 1);
                     break;
@@ -1082,10 +1129,6 @@ __global__ __launch_bounds__(kThreads, MODE == 1 ? 3 : 3) void enum_kernel(EnumP
                 continue;
             }
             backoff = 1;
-            if (idle) {
-                if (g.gl == 0) atomicSub(&p.sc->idle, 1);
-                idle = false;
-            }
             bool ok = jh.q >= 0 && jh.q < p.nq;
             if (ok && jh.q != wk.qid) {
                 flush_query<W>(p, g, wk.qid, qcount, whist);
@@ -1133,7 +1176,6 @@ __global__ __launch_bounds__(kThreads, MODE == 1 ? 3 : 3) void enum_kernel(EnumP
             if (g.gl == 0) atomicSub(&p.sc->outstanding, 1ull);
             if (wk.abort) break;
         }
-        if (idle && g.gl == 0) atomicSub(&p.sc->idle, 1);
         flush_query<W>(p, g, wk.qid, qcount, whist);
         bytes = g.lsum(wk.bytes);
     }
@@ -1166,10 +1208,8 @@ __global__ void enum_init_kernel(SchedCounters *sc, unsigned long long njobs, in
     size_t stride = (size_t)gridDim.x * blockDim.x;
     if (i == 0) {
         sc->outstanding = njobs;
-        sc->idle = 0;
         sc->static_left = (int32_t)(njobs < 0x7fffffffull ? njobs : 0x7fffffffull);
-        sc->q_head = 0;
-        sc->q_tail = 0;
+        sc->q_ht = 0;
         sc->error = 0;
         sc->donations = 0;
         sc->jobs = 0;

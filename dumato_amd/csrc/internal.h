@@ -84,14 +84,12 @@ struct Job {
 struct SchedCounters {
     unsigned long long outstanding;  // unfinished jobs (static + donated)
     char pad0[120];
-    int32_t idle;                    // workers waiting for work
-    char pad1[124];
     int32_t static_left;             // static jobs not yet claimed
     char pad2[124];
-    uint32_t q_head;                 // next ticket of the donated-job queue
-    char pad3[124];
-    uint32_t q_tail;                 // next free slot of the donated-job queue
-    char pad4[124];
+    // donated-job queue: next ticket (high 32 bits) and next free slot (low
+    // 32 bits) in one word, so one load tells whether tickets wait for jobs
+    unsigned long long q_ht;
+    char pad3[120];
     int32_t error;                   // 1 watchdog, 2 arena overflow, 3 bad job
     char pad5[124];
     unsigned long long donations;
@@ -148,7 +146,8 @@ struct EnumParams {
     int32_t pad3;
     unsigned long long *root_cost;    // nullable, per vertex
     unsigned long long *root_count;   // nullable
-    int32_t steal, steal_min, steal_period, bitmap;
+    int32_t steal, steal_min, bitmap, pad4;
+    uint64_t poll_ticks;              // wall-clock ticks between a worker's donation checks
     uint64_t watchdog_ticks;          // wall-clock ticks
 };
 

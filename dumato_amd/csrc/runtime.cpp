@@ -277,8 +277,8 @@ int run_enum(dm_graph *g, int K, bool motif, const dm_config &cfg, uint64_t *his
     } else if (K < 2 || K > kMaxK) {
         return set_error(DM_ERR_ARG, "clique size k=%d unsupported (2..%d)", K, kMaxK);
     }
-    if (cfg.steal_min == 1 || cfg.steal_min < 0 || cfg.steal_period < 1)
-        return set_error(DM_ERR_ARG, "steal_min must be 0 (tuned) or >= 2; steal_period >= 1");
+    if (cfg.steal_min == 1 || cfg.steal_min < 0 || cfg.steal_period < 0)
+        return set_error(DM_ERR_ARG, "steal_min must be 0 (tuned) or >= 2; steal_period >= 0");
     int r = ensure_device(g);
     if (r) return r;
     DeviceGraph *d = g->dev;
@@ -371,7 +371,7 @@ int run_enum(dm_graph *g, int K, bool motif, const dm_config &cfg, uint64_t *his
     p.root_count = want_cnt ? d->d_root_count : nullptr;
     p.steal = cfg.steal ? 1 : 0;
     p.steal_min = cfg.steal_min ? cfg.steal_min : tuned_steal_min(g, K, motif);
-    p.steal_period = cfg.steal_period;
+    p.poll_ticks = (uint64_t)cfg.steal_period * (uint64_t)d->wall_khz / 1000;
     p.bitmap = cfg.clique_bitmap ? 1 : 0;
     p.watchdog_ticks = cfg.watchdog_s > 0 ? (uint64_t)(cfg.watchdog_s * d->wall_khz * 1000.0) : 0;
     if ((r = enum_init_launch(p, (unsigned long long)njobs, d->d_epoch, d->d_results, nres, nullptr, 0, s))) return r;
@@ -455,7 +455,7 @@ int dm_config_init(dm_config *cfg) {
     cfg->vwave = 0;
     cfg->steal = 1;
     cfg->steal_min = 0;
-    cfg->steal_period = 8;
+    cfg->steal_period = 2;
     cfg->lds_bytes = 0;
     cfg->waves_per_block = 4;
     cfg->blocks = 0;

@@ -44,7 +44,8 @@ typedef struct dm_config {
     int32_t steal;           /* 1: idle workers receive donated subtrees */
     int32_t steal_min;       /* donate only levels with >= steal_min children left
                                 (the last level: >= 4 * steal_min); 0 = autotuned */
-    int32_t steal_period;    /* DFS steps between checks for idle workers */
+    int32_t steal_period;    /* microseconds between a worker's checks for idle
+                                workers (0: every DFS step) */
     int32_t lds_bytes;       /* LDS stack bytes per worker; 0 = autotuned */
     int32_t waves_per_block; /* 0 = 4 */
     int32_t blocks;          /* 0 = as many as stay resident on the GPU */
