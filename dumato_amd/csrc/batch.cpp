@@ -251,7 +251,9 @@ static int batch_build(dm_batch *b, const std::vector<dm_query> &qs, const dm_co
     // capture: init kernel + persistent kernel
     hipStream_t s = d->stream;
     HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    int rr = enum_init_launch(p, (unsigned long long)njobs, d->d_epoch, d->d_results, b->nres, b->d_done,
+    int queues = 0;
+    for (size_t w = 0; w + 1 < sq_off.size(); w++) queues += sq_off[w + 1] > sq_off[w] ? 1 : 0;
+    int rr = enum_init_launch(p, (unsigned long long)njobs, queues, d->d_epoch, d->d_results, b->nres, b->d_done,
                               (int)qs.size(), s);
     if (!rr && njobs > 0) rr = enum_launch(p, mode, vw, blocks, 256, lds_total, s);
     hipError_t ce = hipStreamEndCapture(s, &b->graph);
@@ -324,6 +326,15 @@ static int batch_run(dm_batch *b, const std::vector<dm_query> &qs, const dm_conf
     st.jobs = sc1.jobs;
     st.busy_sum_cycles = sc1.busy_sum;
     st.busy_max_cycles = sc1.busy_max;
+    {
+        auto rel = [&](unsigned long long t) {
+            return t > sc1.t_start && sc1.t_start != ~0ull ? (double)(t - sc1.t_start) / d->wall_khz : 0.0;
+        };
+        st.first_job_ms = rel(sc1.t_first_job);
+        st.static_done_ms = rel(sc1.t_static_done);
+        st.last_job_ms = rel(sc1.t_last_job);
+        st.exit_ms = rel(sc1.t_exit);
+    }
     return DM_OK;
 }
 

@@ -43,9 +43,11 @@ namespace dm {
 #define DEV __device__ __forceinline__
 
 constexpr int kThreads = 256;
-// Every instantiation is compiled for at least 3 waves per SIMD (at most 168
-// VGPRs); the full-wave motif kernel fits 128 and so reaches 4.
-constexpr int kMinWaves = 3;
+// Waves per SIMD each instantiation is compiled for: 4 (at most 128 VGPRs)
+// for the full-wave motif kernel, which fits that without spilling, 3 (168
+// VGPRs) for the others.
+template <int W, int MODE>
+constexpr int min_waves() { return W == 64 && MODE == 1 ? 4 : 3; }
 constexpr int kMaxPm = 32;  // 2^(K-2) mask values for K <= 7
 
 // ---------------------------------------------------------------------------
@@ -138,27 +140,31 @@ struct Lvl {
     int32_t sdeg;     // sum of degrees of id[0..l] (algorithmic bytes)
 };
 
-// Per-worker LDS record: Lvl lv[nlv], then uint32 n_gt[kMaxPm] and
-// n_int[kMaxPm] (motif leaf-phase counters), then int32 jid[kJobIds] (the
-// prefix of the current job).  nlv = max(K-2, 1).
+// Per-worker LDS record, at fixed offsets from one base so that a worker
+// holds a single pointer to it: uint64 st[kStats] (the worker's statistics),
+// int32 jid[kJobIds] (the prefix of the current job), uint32 n_gt[kMaxPm]
+// (motif leaf-phase counters), then Lvl lv[nlv], nlv = max(K-2, 1).
+enum { kStBusy, kStJobs, kStFirst, kStStatic, kStLast, kStQCount, kStDonated, kStT0, kStats };
+constexpr int kJidOff = 64;
+constexpr int kNgtOff = kJidOff + kJobIds * 4;
+constexpr int kLvOff = (kNgtOff + kMaxPm * 4 + 15) & ~15;
+static_assert(kStats * 8 <= kJidOff, "worker statistics overlap the job prefix");
 struct WorkerLDS {
-    Lvl *lv;
-    uint32_t *n_gt;
-    uint32_t *n_int;
-    int32_t *jid;
+    char *b;
+    DEV unsigned long long *st() const { return (unsigned long long *)b; }
+    DEV int32_t *jid() const { return (int32_t *)(b + kJidOff); }
+    DEV uint32_t *n_gt() const { return (uint32_t *)(b + kNgtOff); }
+    DEV Lvl *lv() const { return (Lvl *)(b + kLvOff); }
 };
 DEV int nlevels(int K) { return K - 2 > 1 ? K - 2 : 1; }
 __host__ __device__ inline size_t worker_lds_bytes(int K) {
     int nlv = K - 2 > 1 ? K - 2 : 1;
-    return ((size_t)nlv * sizeof(Lvl) + 2 * kMaxPm * sizeof(uint32_t) + kJobIds * 4 + 15) &
-           ~(size_t)15;
+    return ((size_t)kLvOff + (size_t)nlv * sizeof(Lvl) + 15) & ~(size_t)15;
 }
 DEV WorkerLDS worker_lds(char *base, int K) {
+    (void)K;
     WorkerLDS w;
-    w.lv = (Lvl *)base;
-    w.n_gt = (uint32_t *)(w.lv + nlevels(K));
-    w.n_int = w.n_gt + kMaxPm;
-    w.jid = (int32_t *)(w.n_int + kMaxPm);
+    w.b = base;
     return w;
 }
 
@@ -172,15 +178,18 @@ struct Worker {
     int qid;
     bool qmotif;
     uint32_t epoch;
-    char *lds_lo, *lds_top, *lds_hi;
-    char *g_lo, *g_top, *g_hi;
+    char *lds_lo, *lds_top;
+    char *g_lo, *g_top;
     int K;
     int32_t v0;
     int job_lvl;
     int steps;
     unsigned long long cnt;    // per-lane partial count of the current job
     unsigned long long bytes;  // per-lane partial algorithmic bytes
-    unsigned long long donated;
+    // finished jobs not yet subtracted from the shared outstanding count
+    // (lane 0's copy is the one that counts); flushed when the worker runs
+    // out of work, so the count never drops below the unfinished jobs
+    uint32_t pending;
     uint64_t t_start;
     uint64_t last_poll;  // wall clock of the last donation check
     bool abort;
@@ -197,16 +206,15 @@ struct Worker {
                int wid)
         : p(p_), ws(ws_), shist(sh_) {
         lds_lo = lds_top = lds_slice;
-        lds_hi = lds_slice + p.lds_per_worker;
         g_lo = g_top = p.arena + (int64_t)wid * p.arena_per_worker;
-        g_hi = g_lo + p.arena_per_worker;
         K = 0;
         qid = -1;
         qmotif = MODE == 1;
         lut = nullptr;
         epoch = (uint32_t)*p.epoch;
         steps = 0;
-        cnt = bytes = donated = 0;
+        cnt = bytes = 0;
+        pending = 0;
         abort = false;
         t_start = now_ticks();
         last_poll = t_start;
@@ -223,12 +231,12 @@ struct Worker {
 
     DEV char *alloc(int64_t bytes_) {
         int64_t b = (bytes_ + 15) & ~(int64_t)15;
-        if (lds_top + b <= lds_hi) {
+        if (lds_top + b <= lds_lo + p.lds_per_worker) {
             char *r = lds_top;
             lds_top += b;
             return r;
         }
-        if (g_top + b <= g_hi) {
+        if (g_top + b <= g_lo + p.arena_per_worker) {
             char *r = g_top;
             g_top += b;
             return r;
@@ -263,13 +271,17 @@ struct Worker {
             uint32_t t = (uint32_t)__hip_atomic_load(&p.sc->q_ht, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             if (t < p.q_cap) {
                 // count the job before it becomes visible, so outstanding
-                // cannot reach zero while it waits in the queue
-                atomicAdd(&p.sc->outstanding, 1ull);
+                // cannot reach zero while it waits in the queue; a finished
+                // job not yet subtracted from outstanding stands in for it
+                const bool own = pending > 0;
+                if (own) pending--;
+                else atomicAdd(&p.sc->outstanding, 1ull);
                 t = (uint32_t)atomicAdd(&p.sc->q_ht, 1ull);
                 if (t < p.q_cap) {
                     ok = 1;
                     slot = t;
-                    atomicAdd(&p.sc->donations, 1ull);
+                } else if (own) {
+                    pending++;
                 } else {
                     atomicSub(&p.sc->outstanding, 1ull);
                 }
@@ -286,7 +298,7 @@ struct Worker {
             else if (j == 1) v = lo;
             else if (j == 2) v = hi;
             else if (j == 3) v = qid;
-            else v = (j - 4 <= lvl) ? ws.lv[j - 4].vtx : -1;
+            else v = (j - 4 <= lvl) ? ws.lv()[j - 4].vtx : -1;
             __hip_atomic_store(dst + j, tag | (uint32_t)v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
         return true;
@@ -304,7 +316,7 @@ struct Worker {
         const bool clique = MODE == 0 || (MODE == 2 && !qmotif);
         const int e = K - 3 - l;
         if (!clique || e < 1) return half;
-        const float n = (float)ws.lv[l].nu;
+        const float n = (float)ws.lv()[l].nu;
         const float x = 1.0f / (float)(e + 1);
         const float a = powf(n - (float)cur, (float)(e + 1));
         const float b = powf(n - (float)end, (float)(e + 1));
@@ -333,12 +345,12 @@ struct Worker {
         // the last level's children are cheap: split it only in bigger pieces
         const int leaf_min = 4 * p.steal_min;
         for (int l = job_lvl; l < top; l++) {
-            int cur = ws.lv[l].cur, end = ws.lv[l].end;
+            int cur = ws.lv()[l].cur, end = ws.lv()[l].end;
             if (end - cur >= (l == K - 3 ? leaf_min : p.steal_min)) {
                 const int mid = split_point(l, cur, end);
                 if (push_job(l, mid, end)) {
-                    ws.lv[l].end = mid;
-                    donated++;
+                    ws.lv()[l].end = mid;
+                    if (g.gl == 0) ws.st()[kStDonated]++;
                 }
                 return;
             }
@@ -347,14 +359,14 @@ struct Worker {
             int mid = leaf_lo + ((leaf_i + 1 - leaf_lo) >> 1);
             if (push_job(leaf_k, leaf_lo, mid)) {
                 leaf_lo = mid;
-                donated++;
+                if (g.gl == 0) ws.st()[kStDonated]++;
             }
         }
     }
 
     // ---- motif levels ----------------------------------------------------------
     DEV void motif_root_level(int32_t v) {
-        Lvl &L = ws.lv[0];
+        Lvl &L = ws.lv()[0];
         const int32_t *a = p.adj + p.ostart[v];
         int n = (int)(p.off[v + 1] - p.ostart[v]);
         L.uid = const_cast<int32_t *>(a);
@@ -375,8 +387,8 @@ struct Worker {
 
     // U_{l+1}, E_{l+1} from level l and child c (pmc = c's mask in U_l).
     DEV void motif_forward(int l, int32_t c, int pmc) {
-        Lvl &P = ws.lv[l];
-        Lvl &N = ws.lv[l + 1];
+        Lvl &P = ws.lv()[l];
+        Lvl &N = ws.lv()[l + 1];
         const int l1 = l + 1;
         const int32_t *nc = nbr(c);
         const int dc = deg(c);
@@ -467,7 +479,7 @@ struct Worker {
     // (child, neighbour) pairs are flattened across the lanes so short
     // neighbour lists do not leave lanes idle.
     DEV void motif_leaf_phase(int l, int lo, int hi) {
-        Lvl &P = ws.lv[l];
+        Lvl &P = ws.lv()[l];
         const int npm = 1 << (l + 1);  // mask values over the lists 0..l
         const int top = npm;           // bit of the child's own list
         const int s = quick_off(l + 2);
@@ -478,12 +490,12 @@ struct Worker {
         uint32_t *nn = ic + B * npm;
         int32_t *cc = (int32_t *)(nn + B);
         int32_t *pre = cc + B;
-        // ws.n_gt[m]: valid elements of E above the current batch, by mask
-        for (int j = g.gl; j < npm; j += W) ws.n_gt[j] = 0;
+        // ws.n_gt()[m]: valid elements of E above the current batch, by mask
+        for (int j = g.gl; j < npm; j += W) ws.n_gt()[j] = 0;
         for (int j = g.gl; j < B * npm; j += W) ic[j] = 0;
         for (int j = g.gl; j < B; j += W) nn[j] = 0;
         wsync();
-        for (int j = hi + g.gl; j < P.ne; j += W) atomicAdd(&ws.n_gt[epm_at(P, j)], 1u);
+        for (int j = hi + g.gl; j < P.ne; j += W) atomicAdd(&ws.n_gt()[epm_at(P, j)], 1u);
         wsync();
         const int pnu = P.nu;
         const int target = 2 * W;
@@ -531,7 +543,7 @@ struct Worker {
             for (int j = 0; j < npm; j++) {
                 const uint64_t mj = g.ballot(mine && pm_a == j);
                 if (mine) {
-                    const uint32_t gtj = ws.n_gt[j] + (uint32_t)g.below(mj);
+                    const uint32_t gtj = ws.n_gt()[j] + (uint32_t)g.below(mj);
                     const uint32_t ai = ic[g.gl * npm + j];
                     const uint32_t bi = gtj - ai;
                     if (ai) atomicAdd(&shist[lut[indc + ((uint64_t)(j | top) << s)]], (unsigned long long)ai);
@@ -540,7 +552,7 @@ struct Worker {
                     leaves += gtj;
                 }
                 wsync();
-                if (g.gl == 0) ws.n_gt[j] += (uint32_t)__popcll(mj);
+                if (g.gl == 0) ws.n_gt()[j] += (uint32_t)__popcll(mj);
             }
             if (mine) {
                 const uint32_t nw = nn[g.gl];
@@ -557,7 +569,7 @@ struct Worker {
 
     // ---- clique levels ----------------------------------------------------------
     DEV void clique_root_level(int32_t v) {
-        Lvl &L = ws.lv[0];
+        Lvl &L = ws.lv()[0];
         L.uid = const_cast<int32_t *>(p.adj + p.ostart[v]);
         L.nu = (int)(p.off[v + 1] - p.ostart[v]);
         L.vtx = v;
@@ -603,8 +615,8 @@ struct Worker {
     }
 
     DEV void clique_forward(int l, int i) {
-        Lvl &P = ws.lv[l];
-        Lvl &N = ws.lv[l + 1];
+        Lvl &P = ws.lv()[l];
+        Lvl &N = ws.lv()[l + 1];
         const int32_t c = P.uid[i];
         const int32_t *B = P.uid + i + 1;
         const int nb = P.nu - i - 1;
@@ -622,7 +634,7 @@ struct Worker {
     }
 
     DEV void clique_leaf_phase(int l, int lo, int hi) {
-        Lvl &P = ws.lv[l];
+        Lvl &P = ws.lv()[l];
         for (int i = lo; i < hi; i++) {
             if (check_abort()) return;
             int dummy_lo = 0;
@@ -715,7 +727,7 @@ struct Worker {
     DEV void clique_bitmap_root(int32_t v, int lo, int hi, bool first_piece) {
         const int d = odeg(v);
         const int32_t *L = onbr(v);
-        Lvl &R = ws.lv[0];
+        Lvl &R = ws.lv()[0];
         R.uid = const_cast<int32_t *>(L);
         R.nu = d;
         R.vtx = v;
@@ -723,18 +735,41 @@ struct Worker {
         R.mark_glb = g_top;
         if (hi > d) hi = d;
         if (lo >= hi) return;
+        // pair-phase data first, so that it gets the LDS
         uint64_t *rows = (uint64_t *)alloc((int64_t)d * NW * 8);
         int32_t *dp = (int32_t *)alloc((int64_t)d * 4);
-        int32_t *lc = (int32_t *)alloc((int64_t)d * 4);
         int32_t *pre = (int32_t *)alloc((int64_t)(d + 1) * 4);
+        // build scratch, released after the build: a hash of N+(v) (id ->
+        // position, load factor <= 1/2) and each child's list start
+        char *const build_lds = lds_top;
+        char *const build_glb = g_top;
+        constexpr int HB = NW == 1 ? 7 : (NW == 2 ? 8 : 9);
+        constexpr int HS = 1 << HB;
+        int32_t *hkey = (int32_t *)alloc(HS * 4);
+        uint8_t *hval = (uint8_t *)alloc(HS);
+        int64_t *rst = (int64_t *)alloc((int64_t)d * 8);
         if (abort) return;
+        for (int t = g.gl; t < HS; t += W) hkey[t] = -1;
         for (int i = g.gl; i < d; i += W) {
-            int32_t u = L[i];
-            lc[i] = u;
-            dp[i] = odeg(u);
+            const int32_t u = L[i];
+            const int64_t os = p.ostart[u];
+            rst[i] = os;
+            dp[i] = (int)(p.off[u + 1] - os);
         }
         for (int t = lo * NW + g.gl; t < d * NW; t += W) rows[t] = 0;
         wsync();
+        // only children above lo can be set in rows lo.. (lists are sorted)
+        for (int j = lo + g.gl; j < d; j += W) {
+            const int32_t key = L[j];
+            uint32_t h = ((uint32_t)key * 2654435761u) >> (32 - HB);
+            while (true) {
+                if (atomicCAS(&hkey[h], -1, key) == -1) {
+                    hval[h] = (uint8_t)j;
+                    break;
+                }
+                h = (h + 1) & (HS - 1);
+            }
+        }
         // flatten (i, x in N+(L[i])) for i in [lo, d) and set the bits
         int carry = 0;
         for (int base = lo; base < d; base += W) {
@@ -746,22 +781,39 @@ struct Worker {
         }
         if (g.gl == 0) pre[0] = 0;
         wsync();
-        const int nrows = d - lo;
         const int T = carry;
-        for (int t = g.gl; t < T; t += W) {
-            // row r: pre[r] <= t < pre[r+1]
-            int a = 0, b = nrows;
-            while (b - a > 1) {
-                int mid = (a + b) >> 1;
-                if (pre[mid] <= t) a = mid;
-                else b = mid;
+        int a = 0;  // row of this lane's element; t only grows
+        constexpr int U = 4;
+        for (int t0 = 0; t0 < T; t0 += U * W) {
+            int32_t xs[U];
+            int ra[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const int t = t0 + u * W + g.gl;
+                ra[u] = -1;
+                xs[u] = 0;
+                if (t < T) {
+                    while (pre[a + 1] <= t) a++;
+                    ra[u] = a;
+                    xs[u] = p.adj[rst[lo + a] + (t - pre[a])];
+                }
             }
-            const int i = lo + a;
-            const int32_t x = p.adj[p.ostart[lc[i]] + (t - pre[a])];
-            const int j = lower_bound(lc, d, x);
-            if (j < d && lc[j] == x) atomicOr(&rows[i * NW + (j >> 6)], 1ull << (j & 63));
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                if (ra[u] < 0) continue;
+                const int32_t x = xs[u];
+                uint32_t h = ((uint32_t)x * 2654435761u) >> (32 - HB);
+                int k;
+                while ((k = hkey[h]) != x && k != -1) h = (h + 1) & (HS - 1);
+                if (k == x) {
+                    const int j = hval[h];
+                    atomicOr(&rows[(lo + ra[u]) * NW + (j >> 6)], 1ull << (j & 63));
+                }
+            }
         }
         wsync();
+        lds_top = build_lds;
+        g_top = build_glb;
         if (first_piece && g.gl == 0) bytes += 4ull * (unsigned long long)d;
         // first-level children [lo, hi) in batches of W; each batch is
         // flattened into its (i, j) pairs, which the lanes stride over
@@ -820,11 +872,11 @@ struct Worker {
     }
 
     // ---- one job ------------------------------------------------------------
-    // children [lo, hi) of the node whose prefix ws.jid[0..lvl] is in LDS;
+    // children [lo, hi) of the node whose prefix ws.jid()[0..lvl] is in LDS;
     // first_piece: a static job that starts at the root's first child (the
     // root node's own bytes are counted once, by that job)
     DEV void run_job(int lvl, int lo, int hi, bool first_piece) {
-        v0 = ws.jid[0];
+        v0 = ws.jid()[0];
         job_lvl = lvl;
         lds_top = lds_lo;
         g_top = g_lo;
@@ -863,8 +915,8 @@ struct Worker {
         if (first_piece && g.gl == 0)
             bytes += 4ull * (unsigned long long)(motif ? deg(v0) : odeg(v0));
         for (int l = 1; l <= lvl && !abort; l++) {
-            const int32_t c = ws.jid[l];
-            Lvl &P = ws.lv[l - 1];
+            const int32_t c = ws.jid()[l];
+            Lvl &P = ws.lv()[l - 1];
             int pos = lower_bound(P.uid, P.nu, c);
             if (pos >= P.nu || P.uid[pos] != c) {
                 fail(3);
@@ -876,7 +928,7 @@ struct Worker {
         if (abort) return;
         const int leafp = K - 3;
         {
-            Lvl &L = ws.lv[lvl];
+            Lvl &L = ws.lv()[lvl];
             int ne = motif ? L.ne : L.nu;
             L.cur = lo < ne ? lo : ne;
             L.end = hi < ne ? hi : ne;
@@ -884,7 +936,7 @@ struct Worker {
         int k = lvl;
         while (k >= lvl) {
             if (check_abort()) return;
-            Lvl &L = ws.lv[k];
+            Lvl &L = ws.lv()[k];
             if (k == leafp) {
                 if (motif) motif_leaf_phase(k, L.cur, L.end);
                 else clique_leaf_phase(k, L.cur, L.end);
@@ -904,7 +956,7 @@ struct Worker {
                     const int pmc = epm_at(L, i);
                     motif_forward(k, c, pmc);
                     if (abort) return;
-                    Lvl &N = ws.lv[k + 1];
+                    Lvl &N = ws.lv()[k + 1];
                     if (g.gl == 0) bytes += 4ull * (unsigned long long)N.sdeg;
                     N.cur = 0;
                     N.end = N.ne;
@@ -912,7 +964,7 @@ struct Worker {
                     if (g.gl == 0) bytes += 4ull * (unsigned long long)odeg(L.uid[i]);
                     clique_forward(k, i);
                     if (abort) return;
-                    Lvl &N = ws.lv[k + 1];
+                    Lvl &N = ws.lv()[k + 1];
                     N.cur = 0;
                     N.end = N.nu;
                 }
@@ -950,7 +1002,8 @@ DEV bool take_static(const EnumParams &p, const Grp<W> &g, int w, const WorkerLD
                 lo = t[1];
                 hi = t[2];
                 q = t[3];
-                atomicSub(&p.sc->static_left, 1);
+                // whoever claims a queue's last job retires the queue
+                if (b + j == e - 1) atomicSub(&p.sc->static_left, 1);
             }
         }
     }
@@ -971,7 +1024,7 @@ DEV bool take_static(const EnumParams &p, const Grp<W> &g, int w, const WorkerLD
         jh.lo = lo;
         jh.hi = hi;
     }
-    ws.jid[0] = r;
+    ws.jid()[0] = r;
     wsync();
     return true;
 }
@@ -1008,7 +1061,7 @@ DEV bool poll_ticket(const EnumParams &p, const Grp<W> &g, const WorkerLDS &ws, 
 #pragma unroll
     for (int r = 0; r < (kJobWords + W - 1) / W; r++) {
         const int j = g.gl + r * W;
-        if (j >= 4 && j < kJobWords) ws.jid[j - 4] = v[r];
+        if (j >= 4 && j < kJobWords) ws.jid()[j - 4] = v[r];
     }
     jh.lvl = g.bcast(v[0], 0);
     jh.lo = g.bcast(v[0], 1);
@@ -1028,6 +1081,7 @@ DEV bool steal_static(const EnumParams &p, const Grp<W> &g, int w, const WorkerL
     if (g.gl == 0) left = __hip_atomic_load(&p.sc->static_left, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (g.bcast(left, 0) <= 0) {
         static_done = true;
+        if (g.gl == 0) ws.st()[kStStatic] = wall_clock64();
         return false;
     }
     for (int probe = 0; probe < 8; probe++) {
@@ -1043,10 +1097,11 @@ DEV bool steal_static(const EnumParams &p, const Grp<W> &g, int w, const WorkerL
 // (one lane's total) and, in batch launches, the worker's class counters.
 template <int W>
 DEV void flush_query(const EnumParams &p, const Grp<W> &g, int q, unsigned long long count,
-                     unsigned long long *whist) {
+                     unsigned long long *whist, unsigned long long last_ticks) {
     if (q < 0) return;
     const QueryDesc &Q = p.qd[q];
     if (g.gl == 0 && count && !Q.motif) atomicAdd(&Q.out[0], count);
+    if (g.gl == 0 && Q.done_ticks && last_ticks) atomicMax(Q.done_ticks, last_ticks);
     if (p.nq > 1 && Q.motif) {
         for (int i = g.gl; i < Q.ncls; i += W) {
             unsigned long long v = whist[i];
@@ -1061,7 +1116,7 @@ DEV void flush_query(const EnumParams &p, const Grp<W> &g, int q, unsigned long 
 
 // MODE 0: cliques only, 1: motifs only, 2: mixed (batch launches).
 template <int W, int MODE>
-__global__ __launch_bounds__(kThreads, kMinWaves) void enum_kernel(EnumParams p) {
+__global__ __launch_bounds__(kThreads, (min_waves<W, MODE>())) void enum_kernel(EnumParams p) {
     extern __shared__ __align__(16) char smem[];
     Grp<W> g;
     const int wpb = kThreads / W;
@@ -1082,11 +1137,18 @@ __global__ __launch_bounds__(kThreads, kMinWaves) void enum_kernel(EnumParams p)
     char *slice = base + (int64_t)wib * p.lds_per_worker;
     for (int i = threadIdx.x; i < bslots; i += kThreads) bhist[i] = 0;
     for (int i = threadIdx.x; i < wpb * wslots; i += kThreads) ((unsigned long long *)(smem + (size_t)bslots * 8 + wpb * wsb))[i] = 0;
-    if (threadIdx.x == 0 && p.start_ticks) atomicMin(p.start_ticks, (unsigned long long)wall_clock64());
+    if (threadIdx.x == 0) {
+        const unsigned long long t = wall_clock64();
+        atomicMin(&p.sc->t_start, t);
+        if (p.start_ticks) atomicMin(p.start_ticks, t);
+    }
     __syncthreads();
 
-    unsigned long long bytes = 0, jobs = 0, busy = 0, qcount = 0;
+    unsigned long long bytes = 0;
     if (wid < p.nworkers) {
+        if (g.gl == 0)
+            for (int i = 0; i < kStats; i++) ws.st()[i] = 0;
+        wsync();
         Worker<W, MODE> wk(p, ws, single ? bhist : whist, slice, wid);
         uint32_t seed = 0x9E3779B9u * (uint32_t)(wid + 1);
         bool static_done = false, has_ticket = false;
@@ -1103,6 +1165,10 @@ __global__ __launch_bounds__(kThreads, kMinWaves) void enum_kernel(EnumParams p)
             }
             if (!got && static_done) got = poll_ticket<W>(p, g, ws, jh, has_ticket, ticket, wk.epoch);
             if (!got) {
+                if (wk.pending) {
+                    if (g.gl == 0) atomicSub(&p.sc->outstanding, (unsigned long long)wk.pending);
+                    wk.pending = 0;
+                }
                 // the termination words are shared by every worker: read
                 // them on every 4th empty poll only
                 if ((++spins & 3) == 0) {
@@ -1131,8 +1197,8 @@ __global__ __launch_bounds__(kThreads, kMinWaves) void enum_kernel(EnumParams p)
             backoff = 1;
             bool ok = jh.q >= 0 && jh.q < p.nq;
             if (ok && jh.q != wk.qid) {
-                flush_query<W>(p, g, wk.qid, qcount, whist);
-                qcount = 0;
+                if (!single) flush_query<W>(p, g, wk.qid, ws.st()[kStQCount], whist, ws.st()[kStLast]);
+                if (g.gl == 0) ws.st()[kStQCount] = 0;
                 const QueryDesc &Q = p.qd[jh.q];
                 wk.qid = jh.q;
                 wk.K = Q.K;
@@ -1141,7 +1207,7 @@ __global__ __launch_bounds__(kThreads, kMinWaves) void enum_kernel(EnumParams p)
                 ok = (MODE == 2 || (MODE == 1) == wk.qmotif) && Q.K >= 2 && Q.K <= p.kmax;
             }
             const int lvl_max = wk.K == 2 ? 0 : wk.K - 3;
-            const int r0 = ws.jid[0];
+            const int r0 = ws.jid()[0];
             const int r1 = jh.range_end > 0 ? jh.range_end : r0 + 1;
             if (!ok || jh.lvl < 0 || jh.lvl > lvl_max || r0 < 0 || r1 > p.n || r0 >= r1) {
                 wk.fail(3);
@@ -1155,28 +1221,33 @@ __global__ __launch_bounds__(kThreads, kMinWaves) void enum_kernel(EnumParams p)
                                 wk.bytes += 4ull * (unsigned long long)(p.off[root + 1] - p.off[root]);
                             continue;
                         }
-                        ws.jid[0] = root;
+                        ws.jid()[0] = root;
                         wsync();
                     }
-                    const uint64_t c0 = __builtin_amdgcn_s_memtime();
+                    // the job's start time goes through LDS: nothing of the
+                    // bookkeeping stays live in registers across the DFS
+                    if (g.gl == 0) {
+                        if (!ws.st()[kStFirst]) ws.st()[kStFirst] = wall_clock64();
+                        ws.st()[kStT0] = __builtin_amdgcn_s_memtime();
+                    }
                     wk.run_job(jh.lvl, jh.lo, jh.hi, !jh.donated && jh.lvl == 0 && jh.lo == 0);
-                    const uint64_t c1 = __builtin_amdgcn_s_memtime();
-                    busy += c1 - c0;
                     unsigned long long jc = g.lsum(wk.cnt);
                     if (g.gl == 0) {
-                        qcount += jc;
-                        if (p.root_cost) atomicAdd(&p.root_cost[root], (unsigned long long)(c1 - c0));
+                        const unsigned long long dt = __builtin_amdgcn_s_memtime() - ws.st()[kStT0];
+                        ws.st()[kStQCount] += jc;
+                        ws.st()[kStBusy] += dt;
+                        ws.st()[kStLast] = wall_clock64();
+                        if (p.root_cost) atomicAdd(&p.root_cost[root], dt);
                         if (p.root_count && jc) atomicAdd(&p.root_count[root], jc);
                     }
                 }
-                if (g.gl == 0 && p.qd[jh.q].done_ticks)
-                    atomicMax(p.qd[jh.q].done_ticks, (unsigned long long)wall_clock64());
             }
-            jobs++;
-            if (g.gl == 0) atomicSub(&p.sc->outstanding, 1ull);
+            if (g.gl == 0) ws.st()[kStJobs]++;
+            wk.pending++;
             if (wk.abort) break;
         }
-        flush_query<W>(p, g, wk.qid, qcount, whist);
+        wsync();
+        if (!single) flush_query<W>(p, g, wk.qid, ws.st()[kStQCount], whist, ws.st()[kStLast]);
         bytes = g.lsum(wk.bytes);
     }
     __syncthreads();
@@ -1184,11 +1255,44 @@ __global__ __launch_bounds__(kThreads, kMinWaves) void enum_kernel(EnumParams p)
         for (int i = threadIdx.x; i < p.qd[0].ncls; i += kThreads)
             if (bhist[i]) atomicAdd(&p.qd[0].out[i], bhist[i]);
     }
-    if (wid < p.nworkers && g.gl == 0) {
-        if (bytes) atomicAdd(&p.sc->alg_bytes, bytes);
-        if (jobs) atomicAdd(&p.sc->jobs, jobs);
-        atomicAdd(&p.sc->busy_sum, busy);
-        atomicMax(&p.sc->busy_max, busy);
+    // Per-worker statistics (held by lane 0 of each worker) reduced over the
+    // block, so that each block, not each worker, adds to the shared words.
+    constexpr int NF = 9;
+    __shared__ unsigned long long red[kThreads / 64][NF];
+    const bool holder = wid < p.nworkers && g.gl == 0;
+    const unsigned long long *st = ws.st();
+    unsigned long long f[NF] = {holder ? bytes : 0ull,          holder ? st[kStJobs] : 0ull,
+                                holder ? st[kStBusy] : 0ull,    holder ? st[kStDonated] : 0ull,
+                                holder && single ? st[kStQCount] : 0ull,
+                                holder ? st[kStBusy] : 0ull,    holder ? st[kStFirst] : 0ull,
+                                holder ? st[kStStatic] : 0ull,  holder ? st[kStLast] : 0ull};
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+#pragma unroll
+        for (int i = 0; i < NF; i++) {
+            const unsigned long long y = __shfl_xor(f[i], o, 64);
+            f[i] = i < 5 ? f[i] + y : (y > f[i] ? y : f[i]);
+        }
+    }
+    if (__lane_id() == 0) {
+#pragma unroll
+        for (int i = 0; i < NF; i++) red[threadIdx.x / 64][i] = f[i];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < kThreads / 64; w++) {
+            for (int i = 0; i < NF; i++) f[i] = i < 5 ? f[i] + red[w][i] : (red[w][i] > f[i] ? red[w][i] : f[i]);
+        }
+        if (f[0]) atomicAdd(&p.sc->alg_bytes, f[0]);
+        if (f[1]) atomicAdd(&p.sc->jobs, f[1]);
+        if (f[2]) atomicAdd(&p.sc->busy_sum, f[2]);
+        if (f[3]) atomicAdd(&p.sc->donations, f[3]);
+        if (f[4] && single && !p.qd[0].motif) atomicAdd(&p.qd[0].out[0], f[4]);
+        atomicMax(&p.sc->busy_max, f[5]);
+        atomicMax(&p.sc->t_first_job, f[6]);
+        atomicMax(&p.sc->t_static_done, f[7]);
+        atomicMax(&p.sc->t_last_job, f[8]);
+        atomicMax(&p.sc->t_exit, (unsigned long long)wall_clock64());
     }
 }
 
@@ -1201,14 +1305,14 @@ __global__ void fill_u32_kernel(uint32_t *ptr, uint32_t value, size_t count) {
 // Resets the scheduler state of a launch; graph-capturable (no memset or
 // host copies).  Bumps the epoch so queue slots of earlier launches never
 // match.
-__global__ void enum_init_kernel(SchedCounters *sc, unsigned long long njobs, int32_t *epoch, int32_t *sq_cur,
+__global__ void enum_init_kernel(SchedCounters *sc, unsigned long long njobs, int32_t queues, int32_t *epoch, int32_t *sq_cur,
                                  int nworkers, unsigned long long *results, size_t nresults,
                                  unsigned long long *done_ticks, int nq, unsigned long long *start_ticks) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     size_t stride = (size_t)gridDim.x * blockDim.x;
     if (i == 0) {
         sc->outstanding = njobs;
-        sc->static_left = (int32_t)(njobs < 0x7fffffffull ? njobs : 0x7fffffffull);
+        sc->static_left = queues;
         sc->q_ht = 0;
         sc->error = 0;
         sc->donations = 0;
@@ -1217,6 +1321,11 @@ __global__ void enum_init_kernel(SchedCounters *sc, unsigned long long njobs, in
         sc->count = 0;
         sc->busy_sum = 0;
         sc->busy_max = 0;
+        sc->t_start = ~0ull;
+        sc->t_first_job = 0;
+        sc->t_static_done = 0;
+        sc->t_last_job = 0;
+        sc->t_exit = 0;
         *epoch = *epoch + 1;
         if (start_ticks) *start_ticks = ~0ull;
     }
@@ -1281,11 +1390,12 @@ int enum_launch(const EnumParams &p, int mode, int vwave, int blocks, int thread
     return DM_OK;
 }
 
-int enum_init_launch(const EnumParams &p, unsigned long long njobs, int32_t *epoch, unsigned long long *results,
-                     size_t nresults, unsigned long long *done_ticks, int nq, void *stream) {
+int enum_init_launch(const EnumParams &p, unsigned long long njobs, int queues, int32_t *epoch,
+                     unsigned long long *results, size_t nresults, unsigned long long *done_ticks, int nq,
+                     void *stream) {
     size_t work = std::max<size_t>({(size_t)p.nworkers, nresults, (size_t)nq, (size_t)1});
     int blocks = (int)std::min<size_t>((work + 255) / 256, 1024);
-    hipLaunchKernelGGL(enum_init_kernel, dim3(blocks), dim3(256), 0, (hipStream_t)stream, p.sc, njobs, epoch,
+    hipLaunchKernelGGL(enum_init_kernel, dim3(blocks), dim3(256), 0, (hipStream_t)stream, p.sc, njobs, queues, epoch,
                        p.sq_cur, p.nworkers, results, nresults, done_ticks, nq, p.start_ticks);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return set_error(DM_ERR_HIP, "init launch: %s", hipGetErrorString(e));

@@ -84,7 +84,7 @@ struct Job {
 struct SchedCounters {
     unsigned long long outstanding;  // unfinished jobs (static + donated)
     char pad0[120];
-    int32_t static_left;             // static jobs not yet claimed
+    int32_t static_left;             // worker queues whose last job is not yet claimed
     char pad2[124];
     // donated-job queue: next ticket (high 32 bits) and next free slot (low
     // 32 bits) in one word, so one load tells whether tickets wait for jobs
@@ -98,6 +98,10 @@ struct SchedCounters {
     unsigned long long count;        // clique count / motif subgraph total
     unsigned long long busy_sum;     // sum over workers of cycles spent in jobs
     unsigned long long busy_max;     // largest per-worker busy cycles
+    // timeline (wall-clock ticks): first block start, then the latest worker
+    // to start its first job, to find static work exhausted, to end its last
+    // job and to exit
+    unsigned long long t_start, t_first_job, t_static_done, t_last_job, t_exit;
 };
 
 // One counting query of a launch.  A single-graph call has one; a batch
@@ -158,9 +162,10 @@ int enum_occupancy(int vwave, int mode, int threads, size_t lds, int *blocks_per
 int enum_launch(const EnumParams &p, int mode, int vwave, int blocks, int threads, size_t lds,
                 void *stream);
 // resets the scheduler state of a launch (graph-capturable: kernels only)
-int enum_init_launch(const EnumParams &p, unsigned long long njobs, int32_t *epoch,
-                     unsigned long long *results, size_t nresults, unsigned long long *done_ticks,
-                     int nq, void *stream);
+// queues: worker queues holding at least one static job
+int enum_init_launch(const EnumParams &p, unsigned long long njobs, int queues, int32_t *epoch,
+                     unsigned long long *results, size_t nresults, unsigned long long *done_ticks, int nq,
+                     void *stream);
 int fill_u32_launch(void *ptr, uint32_t value, size_t count, void *stream);
 
 }  // namespace dm

@@ -196,10 +196,13 @@ int64_t arena_bound_for(int32_t maxdeg, int32_t max_out_, int K, bool motif) {
     if (motif) {
         for (int l = 1; l <= K - 3; l++) b += 10 * (int64_t)(l + 1) * md + 64;
         b += 4 * md + (int64_t)(K - 2) * md + 64;
+        b += 32 * 8 + 32 * ((int64_t)1 << (K - 2)) * 4 + 32 * 12 + 4 + 16;  // leaf-phase scratch
     } else {
         int64_t mo = std::max<int64_t>(max_out_, 1);
         for (int l = 1; l <= K - 3; l++) b += 4 * mo + 16;
-        b += 256 * 4 * 8 + 3 * 257 * 4 + 64;  // local bitmap of a root (out-degree <= 256)
+        // local bitmap of a root (out-degree <= 256): rows, out-degrees,
+        // prefix, then the build's hash (512 keys + positions) and list starts
+        b += 256 * 4 * 8 + 2 * 257 * 4 + 512 * 5 + 256 * 8 + 6 * 16;
     }
     return (b + 255) & ~(int64_t)255;
 }
@@ -374,7 +377,10 @@ int run_enum(dm_graph *g, int K, bool motif, const dm_config &cfg, uint64_t *his
     p.poll_ticks = (uint64_t)cfg.steal_period * (uint64_t)d->wall_khz / 1000;
     p.bitmap = cfg.clique_bitmap ? 1 : 0;
     p.watchdog_ticks = cfg.watchdog_s > 0 ? (uint64_t)(cfg.watchdog_s * d->wall_khz * 1000.0) : 0;
-    if ((r = enum_init_launch(p, (unsigned long long)njobs, d->d_epoch, d->d_results, nres, nullptr, 0, s))) return r;
+    int queues = 0;
+    for (size_t w = 0; w + 1 < sq_off.size(); w++) queues += sq_off[w + 1] > sq_off[w] ? 1 : 0;
+    if ((r = enum_init_launch(p, (unsigned long long)njobs, queues, d->d_epoch, d->d_results, nres, nullptr, 0, s)))
+        return r;
 
     HIPCHK(hipEventRecord(d->ev0, s));
     if (njobs > 0 && (r = enum_launch(p, motif ? 1 : 0, sh.vwave, sh.blocks, 256, sh.lds_total, s))) return r;
@@ -429,6 +435,15 @@ int run_enum(dm_graph *g, int K, bool motif, const dm_config &cfg, uint64_t *his
     st.jobs = sc1.jobs;
     st.busy_sum_cycles = sc1.busy_sum;
     st.busy_max_cycles = sc1.busy_max;
+    {
+        auto rel = [&](unsigned long long t) {
+            return t > sc1.t_start && sc1.t_start != ~0ull ? (double)(t - sc1.t_start) / d->wall_khz : 0.0;
+        };
+        st.first_job_ms = rel(sc1.t_first_job);
+        st.static_done_ms = rel(sc1.t_static_done);
+        st.last_job_ms = rel(sc1.t_last_job);
+        st.exit_ms = rel(sc1.t_exit);
+    }
     st.vwave = sh.vwave;
     st.workers = sh.workers;
     st.blocks = sh.blocks;

@@ -70,6 +70,10 @@ typedef struct dm_stats {
     uint64_t busy_sum_cycles; /* sum over workers of cycles spent inside jobs */
     uint64_t busy_max_cycles; /* largest per-worker busy cycles (makespan proxy) */
     int32_t vwave, workers, blocks, lds_bytes;
+    /* timeline from the first block's start (ms): the latest worker to start
+     * its first job, to find the static queues empty, to end its last job,
+     * and to exit */
+    double first_job_ms, static_done_ms, last_job_ms, exit_ms;
 } dm_stats;
 
 /* Fills *cfg with the defaults (plain scheduler, autotuned width/LDS). */
