@@ -261,106 +261,122 @@ struct Worker {
     }
 
     // ---- donation --------------------------------------------------------------
-    // Publish a donated job without fences: each of its 16 words is one
-    // 64-bit agent-scope atomic store tagged with the launch epoch in the
+    // Donated jobs are published without fences: each of a job's 16 words is
+    // one 64-bit agent-scope atomic store tagged with the launch epoch in the
     // high half; a reader accepts the slot once all 16 tags match.
-    DEV bool push_job(int lvl, int lo, int hi) {
-        int ok = 0;
-        uint32_t slot = 0;
-        if (g.gl == 0) {
-            uint32_t t = (uint32_t)__hip_atomic_load(&p.sc->q_ht, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (t < p.q_cap) {
-                // count the job before it becomes visible, so outstanding
-                // cannot reach zero while it waits in the queue; a finished
-                // job not yet subtracted from outstanding stands in for it
-                const bool own = pending > 0;
-                if (own) pending--;
-                else atomicAdd(&p.sc->outstanding, 1ull);
-                t = (uint32_t)atomicAdd(&p.sc->q_ht, 1ull);
-                if (t < p.q_cap) {
-                    ok = 1;
-                    slot = t;
-                } else if (own) {
-                    pending++;
-                } else {
-                    atomicSub(&p.sc->outstanding, 1ull);
-                }
-            }
-        }
-        ok = g.bcast(ok, 0);
-        if (!ok) return false;
-        slot = (uint32_t)g.bcast((int)slot, 0);
-        uint64_t *dst = p.q[slot].w;
-        const uint64_t tag = (uint64_t)epoch << 32;
-        for (int j = g.gl; j < kJobWords; j += W) {
-            int32_t v;
-            if (j == 0) v = lvl;
-            else if (j == 1) v = lo;
-            else if (j == 2) v = hi;
-            else if (j == 3) v = qid;
-            else v = (j - 4 <= lvl) ? ws.lv()[j - 4].vtx : -1;
-            __hip_atomic_store(dst + j, tag | (uint32_t)v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
-        return true;
-    }
-
-    // Donate the lowest level's remaining children when workers are idle.
-    // leaf_k / leaf_lo / leaf_i describe the descending leaf-phase loop of
-    // level leaf_k (remaining children [lo, i]); leaf_k < 0 when not in it.
-    // Child index splitting [cur, end) into two halves of equal estimated
-    // work.  A clique child i of a list of n candidates roots about
-    // (n - i)^e subtrees below it, e = K - 3 - l; motif children and the leaf
-    // level are split in half.
-    DEV int split_point(int l, int cur, int end) const {
-        const int half = cur + ((end - cur) >> 1);
+    // Boundary j (0 <= j <= parts) when [cur, end) is cut into `parts`
+    // pieces of equal estimated work.  A clique child i of a list of n
+    // candidates roots about (n - i)^e subtrees, e = K - 3 - l; motif
+    // children and the leaf level count as equal.  Every piece keeps at least
+    // one child (end - cur >= parts).
+    DEV int split_at(int l, int cur, int end, int j, int parts) const {
+        if (j <= 0) return cur;
+        if (j >= parts) return end;
         const bool clique = MODE == 0 || (MODE == 2 && !qmotif);
         const int e = K - 3 - l;
-        if (!clique || e < 1) return half;
-        const float n = (float)ws.lv()[l].nu;
-        const float x = 1.0f / (float)(e + 1);
-        const float a = powf(n - (float)cur, (float)(e + 1));
-        const float b = powf(n - (float)end, (float)(e + 1));
-        int m = (int)(n - powf(0.5f * (a + b), x));
-        if (m <= cur) m = cur + 1;
-        if (m >= end) m = end - 1;
+        int m;
+        if (!clique || e < 1) {
+            m = cur + (int)(((int64_t)(end - cur) * j) / parts);
+        } else {
+            const float n = (float)ws.lv()[l].nu;
+            const float ep = (float)(e + 1);
+            const float a = powf(n - (float)cur, ep);
+            const float b = powf(n - (float)end, ep);
+            const float fr = (float)j / (float)parts;
+            m = (int)(n - powf(a - fr * (a - b), 1.0f / ep));
+        }
+        if (m < cur + j) m = cur + j;
+        if (m > end - (parts - j)) m = end - (parts - j);
         return m;
     }
 
+    // Publishes up to P jobs for the children [cur, end) of level lvl: the
+    // range is cut into np + 1 pieces of equal estimated work, np <= P as the
+    // queue allows, and all but one are queued.  The worker keeps the first
+    // piece (upper == true; returns its new end) or the last one (returns its
+    // new start).  Returns end (start) unchanged when nothing was queued.
+    DEV int push_pieces(int lvl, int cur, int end, int P, bool upper) {
+        int np = 0;
+        uint32_t slot = 0;
+        if (g.gl == 0) {
+            const uint32_t t = (uint32_t)__hip_atomic_load(&p.sc->q_ht, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (t < p.q_cap) {
+                const uint32_t t0 = (uint32_t)atomicAdd(&p.sc->q_ht, (unsigned long long)P);
+                if (t0 < p.q_cap) {
+                    np = (int)(p.q_cap - t0) < P ? (int)(p.q_cap - t0) : P;
+                    slot = t0;
+                    // count the jobs before they become visible, so that
+                    // outstanding cannot reach zero while they wait; finished
+                    // jobs not yet subtracted stand in for some of them
+                    const uint32_t own = pending < (uint32_t)np ? pending : (uint32_t)np;
+                    pending -= own;
+                    if ((uint32_t)np > own) atomicAdd(&p.sc->outstanding, (unsigned long long)(np - own));
+                }
+            }
+        }
+        np = g.bcast(np, 0);
+        if (np == 0) return upper ? end : cur;
+        const int first = upper ? 1 : 0;  // first queued piece
+        slot = (uint32_t)g.bcast((int)slot, 0);
+        const uint64_t tag = (uint64_t)epoch << 32;
+        for (int idx = g.gl; idx < np * kJobWords; idx += W) {
+            const int r = idx / kJobWords, j = idx - r * kJobWords;
+            int32_t v;
+            if (j == 0) v = lvl;
+            else if (j == 1) v = split_at(lvl, cur, end, first + r, np + 1);
+            else if (j == 2) v = split_at(lvl, cur, end, first + r + 1, np + 1);
+            else if (j == 3) v = qid;
+            else v = (j - 4 <= lvl) ? ws.lv()[j - 4].vtx : -1;
+            __hip_atomic_store(p.q[slot + r].w + j, tag | (uint32_t)v, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        }
+        if (g.gl == 0) ws.st()[kStDonated] += (unsigned long long)np;
+        return split_at(lvl, cur, end, upper ? 1 : np, np + 1);
+    }
+
+    // Donate remaining children when tickets wait for jobs: the shallowest
+    // level of the job with enough children left is cut into one piece per
+    // waiting ticket (at most kMaxPieces) plus the piece this worker keeps.
+    // leaf_k / leaf_lo / leaf_i describe the descending leaf-phase loop of
+    // level leaf_k (remaining children [lo, i]); leaf_k < 0 when not in it.
+    static constexpr int kMaxPieces = 8;
     DEV void maybe_donate(int top, int leaf_k, int &leaf_lo, int leaf_i) {
         if (!p.steal) return;
         const uint64_t now = now_ticks();
         if (now - last_poll < p.poll_ticks) return;
         last_poll = now;
-        // donate only when tickets wait for jobs (a worker takes a ticket once
-        // no static job is left).  The check is a hint: a workgroup-scope
-        // load may be served from this CU's cache and lag by a moment, which
-        // only delays a donation; the push itself is exact.
+        // a worker takes a ticket once no static job is left.  The check is a
+        // hint: a workgroup-scope load may be served from this CU's cache and
+        // lag by a moment, which only delays a donation; the push is exact.
         int want = 0;
         if (g.gl == 0) {
             const unsigned long long ht =
                 __hip_atomic_load(&p.sc->q_ht, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-            want = (uint32_t)(ht >> 32) > (uint32_t)ht;
+            const uint32_t h = (uint32_t)(ht >> 32), t = (uint32_t)ht;
+            want = h > t ? (int)(h - t < (uint32_t)kMaxPieces ? h - t : (uint32_t)kMaxPieces) : 0;
         }
-        if (!g.bcast(want, 0)) return;
+        want = g.bcast(want, 0);
+        if (want == 0) return;
         // the last level's children are cheap: split it only in bigger pieces
         const int leaf_min = 4 * p.steal_min;
         for (int l = job_lvl; l < top; l++) {
-            int cur = ws.lv()[l].cur, end = ws.lv()[l].end;
-            if (end - cur >= (l == K - 3 ? leaf_min : p.steal_min)) {
-                const int mid = split_point(l, cur, end);
-                if (push_job(l, mid, end)) {
-                    ws.lv()[l].end = mid;
-                    if (g.gl == 0) ws.st()[kStDonated]++;
-                }
+            const int cur = ws.lv()[l].cur, end = ws.lv()[l].end;
+            const int minc = l == K - 3 ? leaf_min : p.steal_min;
+            if (end - cur >= minc) {
+                int P = (end - cur) / (minc > 1 ? minc / 2 : 1) - 1;
+                if (P > want) P = want;
+                if (P > end - cur - 1) P = end - cur - 1;
+                if (P >= 1) ws.lv()[l].end = push_pieces(l, cur, end, P, true);
                 return;
             }
         }
-        if (leaf_k >= 0 && leaf_i + 1 - leaf_lo >= leaf_min) {
-            int mid = leaf_lo + ((leaf_i + 1 - leaf_lo) >> 1);
-            if (push_job(leaf_k, leaf_lo, mid)) {
-                leaf_lo = mid;
-                if (g.gl == 0) ws.st()[kStDonated]++;
-            }
+        const int left = leaf_i + 1 - leaf_lo;
+        if (leaf_k >= 0 && left >= leaf_min) {
+            // the leaf loop runs downward: queue the lower pieces
+            int P = left / (leaf_min > 1 ? leaf_min / 2 : 1) - 1;
+            if (P > want) P = want;
+            if (P > left - 1) P = left - 1;
+            if (P >= 1) leaf_lo = push_pieces(leaf_k, leaf_lo, leaf_i + 1, P, false);
         }
     }
 
