@@ -190,6 +190,7 @@ struct Worker {
     // (lane 0's copy is the one that counts); flushed when the worker runs
     // out of work, so the count never drops below the unfinished jobs
     uint32_t pending;
+    int range_hi;  // end of the current range job's roots (v0 + 1 otherwise)
     uint64_t t_start;
     uint64_t last_poll;  // wall clock of the last donation check
     bool abort;
@@ -215,6 +216,7 @@ struct Worker {
         steps = 0;
         cnt = bytes = 0;
         pending = 0;
+        range_hi = 0;
         abort = false;
         t_start = now_ticks();
         last_poll = t_start;
@@ -296,28 +298,10 @@ struct Worker {
     // piece (upper == true; returns its new end) or the last one (returns its
     // new start).  Returns end (start) unchanged when nothing was queued.
     DEV int push_pieces(int lvl, int cur, int end, int P, bool upper) {
-        int np = 0;
         uint32_t slot = 0;
-        if (g.gl == 0) {
-            const uint32_t t = (uint32_t)__hip_atomic_load(&p.sc->q_ht, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (t < p.q_cap) {
-                const uint32_t t0 = (uint32_t)atomicAdd(&p.sc->q_ht, (unsigned long long)P);
-                if (t0 < p.q_cap) {
-                    np = (int)(p.q_cap - t0) < P ? (int)(p.q_cap - t0) : P;
-                    slot = t0;
-                    // count the jobs before they become visible, so that
-                    // outstanding cannot reach zero while they wait; finished
-                    // jobs not yet subtracted stand in for some of them
-                    const uint32_t own = pending < (uint32_t)np ? pending : (uint32_t)np;
-                    pending -= own;
-                    if ((uint32_t)np > own) atomicAdd(&p.sc->outstanding, (unsigned long long)(np - own));
-                }
-            }
-        }
-        np = g.bcast(np, 0);
+        const int np = reserve_slots(P, slot);
         if (np == 0) return upper ? end : cur;
         const int first = upper ? 1 : 0;  // first queued piece
-        slot = (uint32_t)g.bcast((int)slot, 0);
         const uint64_t tag = (uint64_t)epoch << 32;
         for (int idx = g.gl; idx < np * kJobWords; idx += W) {
             const int r = idx / kJobWords, j = idx - r * kJobWords;
@@ -332,6 +316,46 @@ struct Worker {
         }
         if (g.gl == 0) ws.st()[kStDonated] += (unsigned long long)np;
         return split_at(lvl, cur, end, upper ? 1 : np, np + 1);
+    }
+
+    // Queues the roots [lo, hi) of this worker's range job as a range job
+    // of their own; returns false when the queue is full.
+    DEV bool push_range(int lo, int hi) {
+        uint32_t slot = 0;
+        if (reserve_slots(1, slot) == 0) return false;
+        const uint64_t tag = (uint64_t)epoch << 32;
+        for (int j = g.gl; j < kJobWords; j += W) {
+            const int32_t v = j == 1 ? -1 : j == 2 ? hi : j == 3 ? qid : j == 4 ? lo : j == 0 ? 0 : -1;
+            __hip_atomic_store(p.q[slot].w + j, tag | (uint32_t)v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        if (g.gl == 0) ws.st()[kStDonated]++;
+        return true;
+    }
+
+    // Reserves up to P consecutive queue slots (as many as the queue has)
+    // and counts the jobs that will fill them as outstanding before they
+    // become visible, so that outstanding cannot reach zero while they wait;
+    // finished jobs not yet subtracted stand in for some of them.  Returns
+    // the number reserved (the same on every lane) and the first slot.
+    DEV int reserve_slots(int P, uint32_t &slot_out) {
+        int np = 0;
+        uint32_t slot = 0;
+        if (g.gl == 0) {
+            const uint32_t t = (uint32_t)__hip_atomic_load(&p.sc->q_ht, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (t < p.q_cap) {
+                const uint32_t t0 = (uint32_t)atomicAdd(&p.sc->q_ht, (unsigned long long)P);
+                if (t0 < p.q_cap) {
+                    np = (int)(p.q_cap - t0) < P ? (int)(p.q_cap - t0) : P;
+                    slot = t0;
+                    const uint32_t own = pending < (uint32_t)np ? pending : (uint32_t)np;
+                    pending -= own;
+                    if ((uint32_t)np > own) atomicAdd(&p.sc->outstanding, (unsigned long long)(np - own));
+                }
+            }
+        }
+        np = g.bcast(np, 0);
+        slot_out = (uint32_t)g.bcast((int)slot, 0);
+        return np;
     }
 
     // Donate remaining children when tickets wait for jobs: the shallowest
@@ -357,13 +381,20 @@ struct Worker {
         }
         want = g.bcast(want, 0);
         if (want == 0) return;
+        // untouched roots of a range job go first: they need no prefix
+        if (range_hi - v0 - 1 >= 2) {
+            const int mid = v0 + 1 + ((range_hi - v0 - 1) >> 1);
+            if (push_range(mid, range_hi)) range_hi = mid;
+            return;
+        }
         // the last level's children are cheap: split it only in bigger pieces
         const int leaf_min = 4 * p.steal_min;
         for (int l = job_lvl; l < top; l++) {
             const int cur = ws.lv()[l].cur, end = ws.lv()[l].end;
             const int minc = l == K - 3 ? leaf_min : p.steal_min;
             if (end - cur >= minc) {
-                int P = (end - cur) / (minc > 1 ? minc / 2 : 1) - 1;
+                int P = (end - cur) / minc - 1;
+                if (P < 1) P = 1;
                 if (P > want) P = want;
                 if (P > end - cur - 1) P = end - cur - 1;
                 if (P >= 1) ws.lv()[l].end = push_pieces(l, cur, end, P, true);
@@ -373,7 +404,8 @@ struct Worker {
         const int left = leaf_i + 1 - leaf_lo;
         if (leaf_k >= 0 && left >= leaf_min) {
             // the leaf loop runs downward: queue the lower pieces
-            int P = left / (leaf_min > 1 ? leaf_min / 2 : 1) - 1;
+            int P = left / leaf_min - 1;
+            if (P < 1) P = 1;
             if (P > want) P = want;
             if (P > left - 1) P = left - 1;
             if (P >= 1) leaf_lo = push_pieces(leaf_k, leaf_lo, leaf_i + 1, P, false);
@@ -1085,6 +1117,11 @@ DEV bool poll_ticket(const EnumParams &p, const Grp<W> &g, const WorkerLDS &ws, 
     jh.q = g.bcast(v[0], 3);
     jh.range_end = 0;
     jh.donated = true;
+    if (jh.lo < 0) {  // the rest of a range job: roots [jid[0], hi)
+        jh.range_end = jh.hi;
+        jh.lo = 0;
+        jh.hi = 0x7fffffff;
+    }
     has_ticket = false;
     wsync();
     return true;
@@ -1228,7 +1265,8 @@ __global__ __launch_bounds__(kThreads, (min_waves<W, MODE>())) void enum_kernel(
             if (!ok || jh.lvl < 0 || jh.lvl > lvl_max || r0 < 0 || r1 > p.n || r0 >= r1) {
                 wk.fail(3);
             } else {
-                for (int root = r0; root < r1 && !wk.abort; root++) {
+                wk.range_hi = r1;
+                for (int root = r0; root < wk.range_hi && !wk.abort; root++) {
                     if (jh.range_end > 0) {
                         // range jobs skip roots without a neighbour above them;
                         // a motif root still counts its own node's bytes
@@ -1246,7 +1284,7 @@ __global__ __launch_bounds__(kThreads, (min_waves<W, MODE>())) void enum_kernel(
                         if (!ws.st()[kStFirst]) ws.st()[kStFirst] = wall_clock64();
                         ws.st()[kStT0] = __builtin_amdgcn_s_memtime();
                     }
-                    wk.run_job(jh.lvl, jh.lo, jh.hi, !jh.donated && jh.lvl == 0 && jh.lo == 0);
+                    wk.run_job(jh.lvl, jh.lo, jh.hi, jh.range_end > 0 || (!jh.donated && jh.lvl == 0 && jh.lo == 0));
                     unsigned long long jc = g.lsum(wk.cnt);
                     if (g.gl == 0) {
                         const unsigned long long dt = __builtin_amdgcn_s_memtime() - ws.st()[kStT0];

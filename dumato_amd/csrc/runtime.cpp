@@ -222,47 +222,63 @@ bool root_has_work(const dm_graph *g, int32_t v, int K, bool motif) {
     return g->rcsr.off[v + 1] - g->ostart[v] > 0;
 }
 
-// Plain: roots in descending degree order (out-degree for cliques), dealt
-// round-robin to the workers like the reference's v -> warp v % W partition
-// (src/main/DuMatoCPU.cu:121-135), heaviest first.  Degree ranks are already
-// in ascending degree order, so motifs need no sort; cliques bucket by
-// out-degree.
+std::vector<int32_t> order_by_cost(const std::vector<StaticJob> &jobs) {
+    const int B = 256;
+    double lmax = -1e300, lmin = 1e300;
+    for (const StaticJob &j : jobs) {
+        const double l = std::log(std::max(j.cost, 1e-300));
+        lmax = std::max(lmax, l);
+        lmin = std::min(lmin, l);
+    }
+    const double scale = lmax > lmin ? (B - 1) / (lmax - lmin) : 0.0;
+    std::vector<int32_t> bkt(jobs.size()), start(B + 1, 0), order(jobs.size());
+    for (size_t i = 0; i < jobs.size(); i++) {
+        const int b = (int)((lmax - std::log(std::max(jobs[i].cost, 1e-300))) * scale);
+        bkt[i] = std::min(std::max(b, 0), B - 1);
+        start[bkt[i] + 1]++;
+    }
+    for (int b = 0; b < B; b++) start[b + 1] += start[b];
+    for (size_t i = 0; i < jobs.size(); i++) order[start[bkt[i]]++] = (int32_t)i;
+    return order;
+}
+
+void write_queues(const std::vector<StaticJob> &jobs, const std::vector<std::vector<int32_t>> &q,
+                  std::vector<int64_t> &sq_off, std::vector<int32_t> &sq_jobs) {
+    const size_t workers = q.size();
+    sq_off.assign(workers + 1, 0);
+    for (size_t w = 0; w < workers; w++) sq_off[w + 1] = sq_off[w] + (int64_t)q[w].size();
+    sq_jobs.assign((size_t)sq_off[workers] * 4, 0);
+    for (size_t w = 0; w < workers; w++) {
+        int64_t o = sq_off[w];
+        for (int32_t ji : q[w]) {
+            sq_jobs[4 * o] = jobs[ji].a;
+            sq_jobs[4 * o + 1] = jobs[ji].b;
+            sq_jobs[4 * o + 2] = jobs[ji].c;
+            sq_jobs[4 * o + 3] = 0;
+            o++;
+        }
+    }
+}
+
+// Plain: one job per root, in descending degree order (out-degree for
+// cliques), dealt round-robin to the workers like the reference's
+// v -> warp v % W partition (src/main/DuMatoCPU.cu:121-135), heaviest first.
+// (Grouping light roots into range jobs measured slower: fewer static jobs
+// end the static phase early and push the work through donations.)
 void plain_partition(const dm_graph *g, const std::vector<int32_t> &roots, int K, bool motif,
                      int workers, std::vector<int64_t> &sq_off, std::vector<int32_t> &sq_jobs) {
     (void)K;
-    std::vector<int32_t> ord;
-    ord.reserve(roots.size());
-    if (motif) {
-        ord.assign(roots.rbegin(), roots.rend());
-    } else {
-        const int32_t mo = g->max_out;
-        std::vector<int64_t> cnt((size_t)mo + 2, 0);
-        for (int32_t v : roots) cnt[mo - (g->rcsr.off[v + 1] - g->ostart[v])]++;
-        int64_t acc = 0;
-        for (auto &c : cnt) {
-            int64_t t = c;
-            c = acc;
-            acc += t;
-        }
-        ord.resize(roots.size());
-        for (int32_t v : roots) ord[cnt[mo - (g->rcsr.off[v + 1] - g->ostart[v])]++] = v;
+    std::vector<StaticJob> jobs(roots.size());
+    for (size_t i = 0; i < roots.size(); i++) {
+        const int32_t v = roots[i];
+        const double d = motif ? (double)(g->rcsr.off[v + 1] - g->rcsr.off[v])
+                               : (double)(g->rcsr.off[v + 1] - g->ostart[v]);
+        jobs[i] = {v, 0, INT_MAX, 1.0 + d};
     }
-    sq_off.assign((size_t)workers + 1, 0);
-    const int64_t n = (int64_t)ord.size();
-    for (int w = 0; w < workers; w++) {
-        int64_t c = n > w ? (n - w + workers - 1) / workers : 0;
-        sq_off[w + 1] = sq_off[w] + c;
-    }
-    sq_jobs.assign((size_t)n * 4, 0);
-    for (int w = 0; w < workers; w++) {
-        int64_t o = sq_off[w];
-        for (int64_t i = w; i < n; i += workers, o++) {
-            sq_jobs[4 * o] = ord[i];
-            sq_jobs[4 * o + 1] = 0;
-            sq_jobs[4 * o + 2] = INT_MAX;
-            sq_jobs[4 * o + 3] = 0;
-        }
-    }
+    const std::vector<int32_t> order = order_by_cost(jobs);
+    std::vector<std::vector<int32_t>> q((size_t)std::max(workers, 1));
+    for (size_t t = 0; t < order.size(); t++) q[t % q.size()].push_back(order[t]);
+    write_queues(jobs, q, sq_off, sq_jobs);
 }
 
 // ---------------------------------------------------------------------------
@@ -290,26 +306,48 @@ int run_enum(dm_graph *g, int K, bool motif, const dm_config &cfg, uint64_t *his
     Shape sh;
     if ((r = pick_shape(g, K, motif, cfg, sh))) return r;
 
-    // roots
+    // roots and their static partition (reused while nothing it depends on
+    // changes)
     const double t_s0 = now_ms();
-    std::vector<int32_t> roots;
-    roots.reserve(n);
     int32_t rb = std::max(cfg.root_begin, 0), re = cfg.root_end > 0 ? std::min(cfg.root_end, n) : n;
-    uint64_t skipped_bytes = 0;  // motif root nodes with an empty extension set
-    for (int32_t v = 0; v < n; v++) {
-        int32_t orig = g->inv[v];
-        if (orig < rb || orig >= re) continue;
-        if (sample_mod > 1 && v % sample_mod != 0) continue;
-        if (root_has_work(g, v, K, motif)) roots.push_back(v);
-        else if (motif) skipped_bytes += 4ull * (uint64_t)(g->rcsr.off[v + 1] - g->rcsr.off[v]);
+    const int learned = cfg.sched == DM_SCHED_LEARNED ? 1 : 0;
+    const int key[7] = {K, motif ? 1 : 0, learned, sh.workers, rb, re, sample_mod};
+    dm_graph::Partition *part = nullptr;
+    for (size_t i = 0; i < g->partitions.size(); i++) {
+        dm_graph::Partition &c = g->partitions[i];
+        if (std::equal(key, key + 7, c.key) && (!learned || c.model_gen == g->model_gen)) {
+            if (i) std::swap(g->partitions[0], c);
+            part = &g->partitions[0];
+            break;
+        }
     }
-    std::vector<int64_t> sq_off;
-    std::vector<int32_t> sq_jobs;
-    if (cfg.sched == DM_SCHED_LEARNED) {
-        if ((r = learned_partition(g, roots, K, motif, cfg, sh, sq_off, sq_jobs))) return r;
-    } else {
-        plain_partition(g, roots, K, motif, sh.workers, sq_off, sq_jobs);
+    if (!part) {
+        std::vector<int32_t> roots;
+        roots.reserve(n);
+        uint64_t skipped_bytes = 0;  // motif root nodes with an empty extension set
+        for (int32_t v = 0; v < n; v++) {
+            int32_t orig = g->inv[v];
+            if (orig < rb || orig >= re) continue;
+            if (sample_mod > 1 && v % sample_mod != 0) continue;
+            if (root_has_work(g, v, K, motif)) roots.push_back(v);
+            else if (motif) skipped_bytes += 4ull * (uint64_t)(g->rcsr.off[v + 1] - g->rcsr.off[v]);
+        }
+        dm_graph::Partition np;
+        std::copy(key, key + 7, np.key);
+        np.model_gen = g->model_gen;
+        np.skipped_bytes = skipped_bytes;
+        if (learned) {
+            if ((r = learned_partition(g, roots, K, motif, cfg, sh, np.sq_off, np.sq_jobs))) return r;
+        } else {
+            plain_partition(g, roots, K, motif, sh.workers, np.sq_off, np.sq_jobs);
+        }
+        if (g->partitions.size() >= 8) g->partitions.pop_back();
+        g->partitions.insert(g->partitions.begin(), std::move(np));
+        part = &g->partitions[0];
     }
+    const std::vector<int64_t> &sq_off = part->sq_off;
+    const std::vector<int32_t> &sq_jobs = part->sq_jobs;
+    const uint64_t skipped_bytes = part->skipped_bytes;
     const double t_s1 = now_ms();
     const int64_t njobs = (int64_t)sq_jobs.size() / 4;
 

@@ -71,6 +71,17 @@ struct dm_graph {
     int tuned_vwave[2][dm::kMaxK + 1] = {{0}};
     int tuned_lds[2][dm::kMaxK + 1] = {{0}};
     int tuned_steal_min[2][dm::kMaxK + 1] = {{0}};
+    // static partitions of recent calls (the roots, costs and model they
+    // derive from do not change between calls); model_gen counts trainings
+    struct Partition {
+        int key[7];
+        uint64_t model_gen;
+        std::vector<int64_t> sq_off;
+        std::vector<int32_t> sq_jobs;
+        uint64_t skipped_bytes;
+    };
+    std::vector<Partition> partitions;  // most recent first
+    uint64_t model_gen = 0;
 };
 
 namespace dm {
@@ -96,6 +107,18 @@ int pick_shape(dm_graph *g, int K, bool motif, const dm_config &cfg, Shape &sh);
 bool root_has_work(const dm_graph *g, int32_t v, int K, bool motif);
 void plain_partition(const dm_graph *g, const std::vector<int32_t> &roots, int K, bool motif, int workers,
                      std::vector<int64_t> &sq_off, std::vector<int32_t> &sq_jobs);
+// A static job: (root, lo, hi) children of a root, or a range of roots
+// (r0, -1, r1); cost in the partition's own units.
+struct StaticJob {
+    int32_t a, b, c;
+    double cost;
+};
+// Indices of jobs in descending cost order (log-bucketed: ties within a
+// bucket keep their input order).
+std::vector<int32_t> order_by_cost(const std::vector<StaticJob> &jobs);
+// Writes per-worker queues: queue w holds the jobs listed in q[w].
+void write_queues(const std::vector<StaticJob> &jobs, const std::vector<std::vector<int32_t>> &q,
+                  std::vector<int64_t> &sq_off, std::vector<int32_t> &sq_jobs);
 int run_enum(dm_graph *g, int K, bool motif, const dm_config &cfg, uint64_t *hist, size_t hist_len,
              uint64_t *count_out, int sample_mod = 1);
 int tuned_steal_min(const dm_graph *g, int K, bool motif);

@@ -366,25 +366,25 @@ int learned_partition(dm_graph *g, const std::vector<int32_t> &roots, int K, boo
     if (r) return r;
     std::vector<float> pred;
     if ((r = predict_all(g, pred))) return r;
-    struct Piece {
-        double cost;
-        int32_t root, lo, hi;
-    };
-    std::vector<Piece> pieces;
-    pieces.reserve(roots.size() * 2);
+    // predicted cost per root; roots above half a worker's share are cut
+    // into child ranges
+    std::vector<double> cost(roots.size());
     double total = 0;
-    for (int32_t v : roots) total += std::exp((double)pred[v]);
+    for (size_t i = 0; i < roots.size(); i++) total += (cost[i] = std::exp((double)pred[roots[i]]));
     const int workers = sh.workers;
     const double target = total / std::max(workers, 1);
     const bool can_split = !(K == 2 && !motif);
-    for (int32_t v : roots) {
-        const double c = std::exp((double)pred[v]);
+    std::vector<StaticJob> jobs;
+    jobs.reserve(roots.size() + 1024);
+    for (size_t ri = 0; ri < roots.size(); ri++) {
+        const int32_t v = roots[ri];
+        const double c = cost[ri];
         const int nch = (int)(g->rcsr.off[v + 1] - g->ostart[v]);  // children of the root
         int parts = 1;
         if (can_split && c > 0.5 * target && nch >= 2)
             parts = (int)std::min<double>(nch, std::ceil(c / (0.5 * target)));
         if (parts <= 1) {
-            pieces.push_back({c, v, 0, INT_MAX});
+            jobs.push_back({v, 0, INT_MAX, c});
             continue;
         }
         // child i of a root carries roughly (nch - i)^e of its work (later
@@ -403,42 +403,24 @@ int learned_partition(dm_graph *g, const std::vector<int32_t> &roots, int K, boo
             if (pidx == parts) {
                 while (hi < nch) a += w[hi++];
             }
-            pieces.push_back({c * (a - acc) / ws, v, lo, hi >= nch ? INT_MAX : hi});
+            jobs.push_back({v, lo, hi >= nch ? INT_MAX : hi, c * (a - acc) / ws});
             acc = a;
             lo = hi;
         }
     }
-    // descending cost order by bucketing log-cost (O(n)); exact order within
-    // a bucket does not matter for the binning below
-    const int B = 4096;
-    double lmax = -1e300, lmin = 1e300;
-    for (const Piece &p : pieces) {
-        double l = std::log(std::max(p.cost, 1e-300));
-        lmax = std::max(lmax, l);
-        lmin = std::min(lmin, l);
-    }
-    const double scale = (lmax > lmin) ? (B - 1) / (lmax - lmin) : 0.0;
-    std::vector<int32_t> bucket_of(pieces.size()), start(B + 1, 0), order(pieces.size());
-    for (size_t i = 0; i < pieces.size(); i++) {
-        double l = std::log(std::max(pieces[i].cost, 1e-300));
-        int bkt = (int)((lmax - l) * scale);
-        bucket_of[i] = std::min(std::max(bkt, 0), B - 1);
-        start[bucket_of[i] + 1]++;
-    }
-    for (int b2 = 0; b2 < B; b2++) start[b2 + 1] += start[b2];
-    for (size_t i = 0; i < pieces.size(); i++) order[start[bucket_of[i]]++] = (int32_t)i;
-    // LPT (least-loaded worker first) for the heaviest pieces, then a snake
+    const std::vector<int32_t> order = order_by_cost(jobs);
+    // LPT (least-loaded worker first) for the heaviest jobs, then a snake
     // deal of the light tail
     using QE = std::pair<double, int>;
     std::priority_queue<QE, std::vector<QE>, std::greater<QE>> heap;
     for (int w = 0; w < workers; w++) heap.push({0.0, w});
-    std::vector<std::vector<int32_t>> q(workers);
+    std::vector<std::vector<int32_t>> q((size_t)std::max(workers, 1));
     const size_t head = std::min(order.size(), (size_t)workers * 2);
     for (size_t t = 0; t < head; t++) {
         QE e = heap.top();
         heap.pop();
         q[e.second].push_back(order[t]);
-        e.first += pieces[order[t]].cost;
+        e.first += jobs[order[t]].cost;
         heap.push(e);
     }
     std::vector<int> by_load(workers);
@@ -451,19 +433,7 @@ int learned_partition(dm_graph *g, const std::vector<int32_t> &roots, int K, boo
         int w = by_load[(lap & 1) ? workers - 1 - pos : pos];
         q[w].push_back(order[t]);
     }
-    sq_off.assign((size_t)workers + 1, 0);
-    for (int w = 0; w < workers; w++) sq_off[w + 1] = sq_off[w] + (int64_t)q[w].size();
-    sq_jobs.assign((size_t)sq_off[workers] * 4, 0);
-    for (int w = 0; w < workers; w++) {
-        int64_t o = sq_off[w];
-        for (int32_t pi : q[w]) {
-            sq_jobs[4 * o] = pieces[pi].root;
-            sq_jobs[4 * o + 1] = pieces[pi].lo;
-            sq_jobs[4 * o + 2] = pieces[pi].hi;
-            sq_jobs[4 * o + 3] = 0;
-            o++;
-        }
-    }
+    write_queues(jobs, q, sq_off, sq_jobs);
     return DM_OK;
 }
 
@@ -530,6 +500,7 @@ int sched_train(dm_graph *g, int mode_motif, int k_train, int epochs, uint64_t s
     M->final_loss = loss;
     free_model(g);
     g->model = M;
+    g->model_gen++;
     if (final_loss) *final_loss = loss;
     return DM_OK;
 }
