@@ -249,7 +249,7 @@ def main():
         line["learned_speedup"] = {"clique_k5_rmat17": line["clique_rmat17"]["k5"]["learned_speedup"],
                                    "clique_k6_rmat17": line["clique_rmat17"]["k6"]["learned_speedup"]}
     if not args.no_batch:
-        line["batch256"] = batch_block(args.warmup, max(3, args.steps // 2))
+        line["batch256"] = batch_block(min(args.warmup, 1), max(1, args.steps // 3))
     if args.full:
         gb = W.ba_100k_planted()
         line["motif5_ba"] = plain_vs_learned(gb, "motif", (5,), 4, 0, 1, budget_ms=2000.0)

@@ -60,6 +60,10 @@ struct dm_batch {
     size_t done_cap = 0;
     unsigned long long *d_start = nullptr;
     size_t start_cap = 0;
+    dm::SchedCounters *d_sc = nullptr;  // one per phase
+    size_t sc_cap = 0;
+    int nphases = 1;
+    int tuned_steal_min = 0;  // from dm_batch_autotune; 0 = not tuned
     // cached launch
     dm::BatchPlan plan;
     hipGraph_t graph = nullptr;
@@ -137,9 +141,14 @@ static void batch_partition(const dm_batch *b, const std::vector<dm_query> &qs, 
     }
 }
 
+// The launch is captured as one hipGraph.  A batch that mixes clique and
+// motif queries runs one persistent kernel per family inside it (clique
+// queries first), so that each family gets its specialised kernel and
+// occupancy; the two share the queue, the result slots and the clock.
 static int batch_build(dm_batch *b, const std::vector<dm_query> &qs, const dm_config &cfg) {
     DeviceGraph *d = b->dev;
-    int kmax = 2, hslots = 0, mode = -1;
+    int kmax_f[2] = {2, 2}, hslots = 0;
+    bool has[2] = {false, false};
     for (const dm_query &Q : qs) {
         if (Q.graph < 0 || Q.graph >= b->ngraphs) return set_error(DM_ERR_ARG, "query graph %d out of range", Q.graph);
         if (Q.motif) {
@@ -150,33 +159,62 @@ static int batch_build(dm_batch *b, const std::vector<dm_query> &qs, const dm_co
         } else if (Q.k < 2 || Q.k > kMaxK) {
             return set_error(DM_ERR_ARG, "batch clique k=%d (2..%d)", Q.k, kMaxK);
         }
-        kmax = std::max(kmax, Q.k);
-        int m = Q.motif ? 1 : 0;
-        mode = mode < 0 ? m : (mode == m ? m : 2);
+        const int f = Q.motif ? 1 : 0;
+        has[f] = true;
+        kmax_f[f] = std::max(kmax_f[f], Q.k);
     }
-    if (qs.size() == 1) mode = 2;  // one query still uses the batch counters
     // launch shape: width / LDS from the config (autotuned values cached in the plan)
     int vw = cfg.vwave ? cfg.vwave : (b->plan.vwave ? b->plan.vwave : 64);
     if (vw != 8 && vw != 16 && vw != 32 && vw != 64) return set_error(DM_ERR_ARG, "vwave must be 8/16/32/64");
     int lds = cfg.lds_bytes > 0 ? cfg.lds_bytes : 32768 / (256 / vw);
     lds = (lds + 15) & ~15;
-    const int sm = cfg.steal_min ? cfg.steal_min : 2;
+    const int sm = cfg.steal_min ? cfg.steal_min : (b->tuned_steal_min ? b->tuned_steal_min : 2);
     if (b->exec && b->plan.same(qs, cfg, vw, lds, sm)) return DM_OK;
     drop_graph(b);
-    // one query: class counters shared per block; several: per worker
-    const int ncls_single = qs.size() == 1 && qs[0].motif ? hslots : 0;
-    const int wslots = qs.size() > 1 ? hslots : 0;
-    size_t lds_total = enum_lds_bytes(vw, 256, kmax, ncls_single, wslots, lds);
-    int bpc = 0;
-    int r = enum_occupancy(vw, mode, 256, lds_total, &bpc);
-    if (r) return r;
-    if (bpc < 1) return set_error(DM_ERR_ARG, "batch launch does not fit one CU (LDS %zu B)", lds_total);
-    const int blocks = cfg.blocks > 0 ? cfg.blocks : bpc * d->num_cu;
-    const int workers = blocks * (256 / vw);
-    std::vector<int64_t> sq_off;
-    std::vector<int32_t> sq_jobs;
-    batch_partition(b, qs, workers, sq_off, sq_jobs);
-    const int64_t njobs = (int64_t)sq_jobs.size() / 4;
+    // phases: (mode, kmax, per-worker class slots) per family present; a
+    // single query keeps the batch's per-worker counters (mode 2)
+    struct Phase {
+        int mode, kmax, wslots, blocks, workers;
+        size_t lds_total;
+        std::vector<int64_t> sq_off;
+        std::vector<int32_t> sq_jobs;
+        std::vector<dm_query> qs;     // the family's queries
+        std::vector<int32_t> qidx;    // their indices in qs
+    };
+    std::vector<Phase> ph;
+    for (int f = 0; f < 2; f++) {
+        if (!has[f]) continue;
+        Phase P;
+        P.mode = qs.size() == 1 ? 2 : f;
+        P.kmax = kmax_f[f];
+        P.wslots = f == 1 ? hslots : 0;
+        for (size_t i = 0; i < qs.size(); i++)
+            if ((qs[i].motif ? 1 : 0) == f) {
+                P.qs.push_back(qs[i]);
+                P.qidx.push_back((int32_t)i);
+            }
+        ph.push_back(std::move(P));
+    }
+    int r = DM_OK;
+    int max_workers = 0;
+    size_t sq_total = 0, jobs_total = 0;
+    for (Phase &P : ph) {
+        // one query: the kernel keeps block-wide class counters (sized by
+        // the query's classes) instead of per-worker ones
+        const int ncls_single = qs.size() == 1 ? P.wslots : 0;
+        P.lds_total = enum_lds_bytes(vw, 256, P.kmax, ncls_single, qs.size() == 1 ? 0 : P.wslots, lds);
+        int bpc = 0;
+        if ((r = enum_occupancy(vw, P.mode, 256, P.lds_total, &bpc))) return r;
+        if (bpc < 1) return set_error(DM_ERR_ARG, "batch launch does not fit one CU (LDS %zu B)", P.lds_total);
+        P.blocks = cfg.blocks > 0 ? cfg.blocks : bpc * d->num_cu;
+        P.workers = P.blocks * (256 / vw);
+        batch_partition(b, P.qs, P.workers, P.sq_off, P.sq_jobs);
+        // job records name queries by their index in the whole batch
+        for (size_t j = 0; j < P.sq_jobs.size(); j += 4) P.sq_jobs[j + 3] = P.qidx[P.sq_jobs[j + 3]];
+        max_workers = std::max(max_workers, P.workers);
+        sq_total += P.sq_off.size();
+        jobs_total += P.sq_jobs.size();
+    }
     // result slots
     b->res_off.assign(qs.size() + 1, 0);
     for (size_t i = 0; i < qs.size(); i++)
@@ -185,15 +223,16 @@ static int batch_build(dm_batch *b, const std::vector<dm_query> &qs, const dm_co
     int64_t arena = 256;
     for (const dm_query &Q : qs)
         arena = std::max(arena, arena_bound_for(b->u.maxdeg, b->max_out, Q.k, Q.motif != 0));
-    if ((r = dev_grow(d->d_arena, d->arena_cap, (size_t)arena * workers))) return r;
-    if ((r = dev_grow(d->d_sq_off, d->sq_off_cap, sq_off.size()))) return r;
-    if ((r = dev_grow(d->d_sq_jobs, d->sq_jobs_cap, std::max<size_t>(sq_jobs.size(), 4)))) return r;
-    if ((r = dev_grow(d->d_sq_cur, d->sq_cur_cap, (size_t)workers))) return r;
+    if ((r = dev_grow(d->d_arena, d->arena_cap, (size_t)arena * max_workers))) return r;
+    if ((r = dev_grow(d->d_sq_off, d->sq_off_cap, sq_total))) return r;
+    if ((r = dev_grow(d->d_sq_jobs, d->sq_jobs_cap, std::max<size_t>(jobs_total, 4)))) return r;
+    if ((r = dev_grow(d->d_sq_cur, d->sq_cur_cap, (size_t)max_workers))) return r;
     if ((r = ensure_queue(d))) return r;
     if ((r = dev_grow(d->d_results, d->results_cap, b->nres))) return r;
     if ((r = dev_grow(b->d_qd, b->qd_cap, qs.size()))) return r;
     if ((r = dev_grow(b->d_done, b->done_cap, qs.size()))) return r;
     if ((r = dev_grow(b->d_start, b->start_cap, 1))) return r;
+    if ((r = dev_grow(b->d_sc, b->sc_cap, 2))) return r;
     std::vector<QueryDesc> qd(qs.size());
     b->q_graph.assign(qs.size(), 0);
     for (size_t i = 0; i < qs.size(); i++) {
@@ -218,48 +257,64 @@ static int batch_build(dm_batch *b, const std::vector<dm_query> &qs, const dm_co
         D.done_ticks = b->d_done + i;
     }
     HIPCHK(hipMemcpy(b->d_qd, qd.data(), qd.size() * sizeof(QueryDesc), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(d->d_sq_off, sq_off.data(), sq_off.size() * 8, hipMemcpyHostToDevice));
-    if (!sq_jobs.empty()) HIPCHK(hipMemcpy(d->d_sq_jobs, sq_jobs.data(), sq_jobs.size() * 4, hipMemcpyHostToDevice));
-
-    EnumParams p;
-    memset(&p, 0, sizeof(p));
-    p.off = d->d_off;
-    p.adj = d->d_adj;
-    p.ostart = d->d_ostart;
-    p.n = b->u.n;
-    p.nq = (int32_t)qs.size();
-    p.qd = b->d_qd;
-    p.kmax = kmax;
-    p.hist_slots = wslots;
-    p.nworkers = workers;
-    p.sq_off = d->d_sq_off;
-    p.sq_jobs = d->d_sq_jobs;
-    p.sq_cur = d->d_sq_cur;
-    p.q = d->d_q;
-    p.q_cap = (uint32_t)d->q_cap_alloc;
-    p.epoch = d->d_epoch;
-    p.start_ticks = b->d_start;
-    p.sc = d->d_sc;
-    p.arena = d->d_arena;
-    p.arena_per_worker = arena;
-    p.lds_per_worker = lds;
-    p.steal = cfg.steal ? 1 : 0;
-    p.steal_min = sm;
-    p.poll_ticks = (uint64_t)(cfg.steal_period >= 0 ? cfg.steal_period : 2) * (uint64_t)d->wall_khz / 1000;
-    p.bitmap = cfg.clique_bitmap ? 1 : 0;
-    p.watchdog_ticks = cfg.watchdog_s > 0 ? (uint64_t)(cfg.watchdog_s * d->wall_khz * 1000.0) : 0;
-    // capture: init kernel + persistent kernel
+    size_t o_off = 0, o_jobs = 0;
+    std::vector<EnumParams> params(ph.size());
+    for (size_t pi = 0; pi < ph.size(); pi++) {
+        Phase &P = ph[pi];
+        HIPCHK(hipMemcpy(d->d_sq_off + o_off, P.sq_off.data(), P.sq_off.size() * 8, hipMemcpyHostToDevice));
+        if (!P.sq_jobs.empty())
+            HIPCHK(hipMemcpy(d->d_sq_jobs + o_jobs, P.sq_jobs.data(), P.sq_jobs.size() * 4, hipMemcpyHostToDevice));
+        EnumParams &p = params[pi];
+        memset(&p, 0, sizeof(p));
+        p.off = d->d_off;
+        p.adj = d->d_adj;
+        p.ostart = d->d_ostart;
+        p.n = b->u.n;
+        p.nq = (int32_t)qs.size();
+        p.qd = b->d_qd;
+        p.kmax = P.kmax;
+        p.hist_slots = P.wslots;
+        p.nworkers = P.workers;
+        p.sq_off = d->d_sq_off + o_off;
+        p.sq_jobs = d->d_sq_jobs + o_jobs;
+        p.sq_cur = d->d_sq_cur;
+        p.q = d->d_q;
+        p.q_cap = (uint32_t)d->q_cap_alloc;
+        p.epoch = d->d_epoch;
+        p.start_ticks = b->d_start;
+        p.sc = b->d_sc + pi;
+        p.arena = d->d_arena;
+        p.arena_per_worker = arena;
+        p.lds_per_worker = lds;
+        p.steal = cfg.steal ? 1 : 0;
+        p.steal_min = sm;
+        p.poll_ticks = (uint64_t)(cfg.steal_period >= 0 ? cfg.steal_period : 2) * (uint64_t)d->wall_khz / 1000;
+        p.bitmap = cfg.clique_bitmap ? 1 : 0;
+        p.watchdog_ticks = cfg.watchdog_s > 0 ? (uint64_t)(cfg.watchdog_s * d->wall_khz * 1000.0) : 0;
+        o_off += P.sq_off.size();
+        o_jobs += P.sq_jobs.size();
+    }
+    // capture: per phase an init kernel and the persistent kernel; only the
+    // first init clears the results, latencies and start clock
     hipStream_t s = d->stream;
     HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    int queues = 0;
-    for (size_t w = 0; w + 1 < sq_off.size(); w++) queues += sq_off[w + 1] > sq_off[w] ? 1 : 0;
-    int rr = enum_init_launch(p, (unsigned long long)njobs, queues, d->d_epoch, d->d_results, b->nres, b->d_done,
-                              (int)qs.size(), s);
-    if (!rr && njobs > 0) rr = enum_launch(p, mode, vw, blocks, 256, lds_total, s);
+    int rr = DM_OK;
+    for (size_t pi = 0; pi < ph.size() && !rr; pi++) {
+        const Phase &P = ph[pi];
+        const int64_t njobs = (int64_t)P.sq_jobs.size() / 4;
+        int queues = 0;
+        for (size_t w = 0; w + 1 < P.sq_off.size(); w++) queues += P.sq_off[w + 1] > P.sq_off[w] ? 1 : 0;
+        EnumParams ip = params[pi];
+        if (pi > 0) ip.start_ticks = nullptr;
+        rr = enum_init_launch(ip, (unsigned long long)njobs, queues, d->d_epoch, pi == 0 ? d->d_results : nullptr,
+                              pi == 0 ? b->nres : 0, pi == 0 ? b->d_done : nullptr, pi == 0 ? (int)qs.size() : 0, s);
+        if (!rr && njobs > 0) rr = enum_launch(params[pi], P.mode, vw, P.blocks, 256, P.lds_total, s);
+    }
     hipError_t ce = hipStreamEndCapture(s, &b->graph);
     if (rr) return rr;
     if (ce != hipSuccess) return set_error(DM_ERR_HIP, "graph capture: %s", hipGetErrorString(ce));
     HIPCHK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    b->nphases = (int)ph.size();
     b->plan.queries = qs;
     b->plan.vwave = vw;
     b->plan.lds = lds;
@@ -270,8 +325,8 @@ static int batch_build(dm_batch *b, const std::vector<dm_query> &qs, const dm_co
     b->plan.blocks = cfg.blocks;
     b->plan.watchdog_s = cfg.watchdog_s;
     b->last.vwave = vw;
-    b->last.workers = workers;
-    b->last.blocks = blocks;
+    b->last.workers = max_workers;
+    b->last.blocks = ph.back().blocks;
     b->last.lds_bytes = lds;
     return DM_OK;
 }
@@ -289,8 +344,8 @@ static int batch_run(dm_batch *b, const std::vector<dm_query> &qs, const dm_conf
     HIPCHK(hipEventRecord(d->ev0, s));
     HIPCHK(hipGraphLaunch(b->exec, s));
     HIPCHK(hipEventRecord(d->ev1, s));
-    SchedCounters sc1;
-    HIPCHK(hipMemcpyAsync(&sc1, d->d_sc, sizeof(sc1), hipMemcpyDeviceToHost, s));
+    SchedCounters scs[2];
+    HIPCHK(hipMemcpyAsync(scs, b->d_sc, sizeof(SchedCounters) * b->nphases, hipMemcpyDeviceToHost, s));
     std::vector<unsigned long long> res(b->nres), done(qs.size());
     unsigned long long start = 0;
     HIPCHK(hipMemcpyAsync(res.data(), d->d_results, b->nres * 8, hipMemcpyDeviceToHost, s));
@@ -299,6 +354,22 @@ static int batch_run(dm_batch *b, const std::vector<dm_query> &qs, const dm_conf
     HIPCHK(hipStreamSynchronize(s));
     float kms = 0;
     HIPCHK(hipEventElapsedTime(&kms, d->ev0, d->ev1));
+    // both phases' counters: sums, maxima, and the timeline of the last phase
+    // measured from the first one's start
+    SchedCounters sc1 = scs[0];
+    for (int pi = 1; pi < b->nphases; pi++) {
+        const SchedCounters &c = scs[pi];
+        if (!sc1.error) sc1.error = c.error;
+        sc1.alg_bytes += c.alg_bytes;
+        sc1.donations += c.donations;
+        sc1.jobs += c.jobs;
+        sc1.busy_sum += c.busy_sum;
+        sc1.busy_max = std::max(sc1.busy_max, c.busy_max);
+        sc1.t_first_job = c.t_first_job;
+        sc1.t_static_done = c.t_static_done;
+        sc1.t_last_job = c.t_last_job;
+        sc1.t_exit = c.t_exit;
+    }
     if (sc1.error) {
         const char *why = sc1.error == 1 ? "watchdog expired" : sc1.error == 2 ? "stack arena overflow" : "invalid job";
         return set_error(DM_ERR_DEVICE, "batch kernel aborted: %s", why);
@@ -396,33 +467,74 @@ int dm_batch_run(dm_batch *b, const dm_query *queries, size_t nq, const dm_confi
     return batch_run(b, qs, c, results, results_len, result_offsets, graph_latency_ms);
 }
 
+// Tunes width and steal granularity on a sample of the queries: every S-th
+// query in descending order of estimated cost, S doubled until one trial
+// takes at most a twelfth of the budget, then as many of the candidates as
+// the budget allows.
 int dm_batch_autotune(dm_batch *b, const dm_query *queries, size_t nq, double budget_ms, dm_config *best) {
     if (!b || !queries || nq == 0) return set_error(DM_ERR_ARG, "need batch and queries");
-    std::vector<dm_query> qs(queries, queries + nq);
+    std::vector<dm_query> all(queries, queries + nq);
+    for (const dm_query &Q : all)
+        if (Q.graph < 0 || Q.graph >= b->ngraphs) return set_error(DM_ERR_ARG, "query graph %d out of range", Q.graph);
+    std::vector<double> est(nq, 0.0);
+    for (size_t i = 0; i < nq; i++) {
+        const dm_query &Q = all[i];
+        const int32_t v0 = b->vbase[Q.graph], v1 = v0 + b->gn[Q.graph];
+        for (int32_t v = v0; v < v1; v++) {
+            const double d = Q.motif ? (double)(b->u.off[v + 1] - b->u.off[v]) : (double)(b->u.off[v + 1] - b->ostart[v]);
+            est[i] += std::pow(d, std::max(1, Q.k - 2));
+        }
+    }
+    std::vector<size_t> ord(nq);
+    for (size_t i = 0; i < nq; i++) ord[i] = i;
+    std::stable_sort(ord.begin(), ord.end(), [&](size_t a, size_t c) { return est[a] > est[c]; });
+    auto sample = [&](size_t S) {
+        std::vector<dm_query> q;
+        for (size_t i = S / 2; i < nq; i += S) q.push_back(all[ord[i]]);
+        if (q.empty()) q.push_back(all[ord[nq - 1]]);
+        return q;
+    };
     dm_config base;
     dm_config_init(&base);
-    double t_best = 1e300;
-    dm_config bc = base;
+    base.vwave = 64;
+    base.steal_min = 2;
+    auto trial = [&](const std::vector<dm_query> &q, const dm_config &c, double &ms) -> int {
+        int r = batch_run(b, q, c, nullptr, 0, nullptr, nullptr);  // capture (and a first run)
+        if (r) return r;
+        r = batch_run(b, q, c, nullptr, 0, nullptr, nullptr);  // replay timing
+        ms = b->last.device_ms;
+        return r;
+    };
     const double t_start = now_ms();
+    size_t S = std::max<size_t>(1, (nq + 47) / 48);
+    std::vector<dm_query> qs = sample(S);
+    double t_best = 0;
+    int r = trial(qs, base, t_best);
+    if (r) return r;
+    while (t_best > budget_ms / 12 && qs.size() > 4 && now_ms() - t_start < budget_ms) {
+        S *= 2;
+        qs = sample(S);
+        if ((r = trial(qs, base, t_best))) return r;
+    }
+    dm_config bc = base;
     for (int w : {64, 32, 16}) {
         for (int sm : {2, 8}) {
+            if (w == 64 && sm == 2) continue;
+            if (now_ms() - t_start > budget_ms) break;
             dm_config c = base;
             c.vwave = w;
             c.steal_min = sm;
-            int r = batch_run(b, qs, c, nullptr, 0, nullptr, nullptr);
-            if (r) return r;
-            r = batch_run(b, qs, c, nullptr, 0, nullptr, nullptr);  // replay timing
-            if (r) return r;
-            if (b->last.device_ms < t_best) {
-                t_best = b->last.device_ms;
+            double t = 0;
+            if ((r = trial(qs, c, t))) return r;
+            if (t < t_best) {
+                t_best = t;
                 bc = c;
             }
-            if (now_ms() - t_start > budget_ms) break;
         }
-        if (now_ms() - t_start > budget_ms) break;
     }
     bc.lds_bytes = 32768 / (256 / bc.vwave);
     b->plan.vwave = bc.vwave;
+    b->tuned_steal_min = bc.steal_min;
     if (best) *best = bc;
     return DM_OK;
 }
@@ -455,6 +567,7 @@ void dm_batch_free(dm_batch *b) {
     if (b->d_qd) (void)hipFree(b->d_qd);
     if (b->d_done) (void)hipFree(b->d_done);
     if (b->d_start) (void)hipFree(b->d_start);
+    if (b->d_sc) (void)hipFree(b->d_sc);
     if (b->dev) {
         dm_graph tmp;
         tmp.dev = b->dev;
