@@ -197,9 +197,14 @@ struct Worker {
     // motif leaf-phase scratch, reserved first in the LDS slice of each job:
     // cs int64[B] | ic u32[B][npm] | nn u32[B] | cc i32[B] | pre i32[B + 1]
     char *lf;
-    static constexpr int kLeafSlots = W < 32 ? W : 32;
+    // children per leaf batch: a full worker, capped so that the per-child
+    // mask counters (slots x 2^(K-2)) stay within 512 words
+    DEV static int leaf_slots(int K) {
+        const int cap = 512 >> (K - 2);
+        return W < cap ? W : cap;
+    }
     DEV static int64_t leaf_scratch_bytes(int K) {
-        const int B = kLeafSlots;
+        const int B = leaf_slots(K);
         return (int64_t)B * 8 + (int64_t)B * (1 << (K - 2)) * 4 + (int64_t)B * 12 + 4;
     }
 
@@ -532,7 +537,7 @@ struct Worker {
         const int top = npm;           // bit of the child's own list
         const int s = quick_off(l + 2);
         const int o1 = quick_off(l + 1);
-        constexpr int B = kLeafSlots;
+        const int B = leaf_slots(K);
         int64_t *cs = (int64_t *)lf;
         uint32_t *ic = (uint32_t *)(cs + B);
         uint32_t *nn = ic + B * npm;
@@ -546,7 +551,10 @@ struct Worker {
         for (int j = hi + g.gl; j < P.ne; j += W) atomicAdd(&ws.n_gt()[epm_at(P, j)], 1u);
         wsync();
         const int pnu = P.nu;
-        const int target = 2 * W;
+        // a batch takes children until their neighbour lists hold about
+        // `target` elements: few, large batches amortise the per-batch
+        // class bookkeeping
+        const int target = 32 * W;
         int i = hi - 1;
         while (i >= lo) {
             if (check_abort()) return;

@@ -196,7 +196,7 @@ int64_t arena_bound_for(int32_t maxdeg, int32_t max_out_, int K, bool motif) {
     if (motif) {
         for (int l = 1; l <= K - 3; l++) b += 10 * (int64_t)(l + 1) * md + 64;
         b += 4 * md + (int64_t)(K - 2) * md + 64;
-        b += 32 * 8 + 32 * ((int64_t)1 << (K - 2)) * 4 + 32 * 12 + 4 + 16;  // leaf-phase scratch
+        b += 64 * 8 + 512 * 4 + 64 * 12 + 4 + 16;  // leaf-phase scratch (at most 64 slots, 512 counters)
     } else {
         int64_t mo = std::max<int64_t>(max_out_, 1);
         for (int l = 1; l <= K - 3; l++) b += 4 * mo + 16;
