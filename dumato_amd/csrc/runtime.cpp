@@ -164,7 +164,25 @@ int ensure_device(dm_graph *g) {
 
 // ---------------------------------------------------------------------------
 // launch shape
-static int default_lds(int vwave) { return 32768 / (256 / vwave); }
+// Largest per-worker LDS stack (a multiple of 256 B, at most 16 KiB) that
+// keeps the occupancy the kernel's registers allow: the LDS of a CU (160
+// KiB) divided over that many blocks, less each block's fixed records.
+int max_lds_per_worker(int vwave, int mode, int K, int ncls_single, int hist_slots) {
+    int bpc = 0;
+    const size_t fixed = enum_lds_bytes(vwave, 256, K, ncls_single, hist_slots, 0);
+    if (enum_occupancy(vwave, mode, 256, fixed, &bpc) != DM_OK || bpc < 1) return 32768 / (256 / vwave);
+    const size_t per_block = (size_t)160 * 1024 / (size_t)bpc;
+    if (per_block <= fixed + 256) return 32768 / (256 / vwave);
+    int lds = std::min((int)(((per_block - fixed) / (size_t)(256 / vwave)) & ~(size_t)255), 16384);
+    // the LDS allocation granularity may cost a block: step down until the
+    // occupancy holds
+    for (; lds > 256; lds -= 256) {
+        int b2 = 0;
+        const size_t total = enum_lds_bytes(vwave, 256, K, ncls_single, hist_slots, lds);
+        if (enum_occupancy(vwave, mode, 256, total, &b2) == DM_OK && b2 >= bpc) break;
+    }
+    return lds;
+}
 
 int pick_shape(dm_graph *g, int K, bool motif, const dm_config &cfg, Shape &sh) {
     DeviceGraph *d = g->dev;
@@ -172,10 +190,11 @@ int pick_shape(dm_graph *g, int K, bool motif, const dm_config &cfg, Shape &sh) 
     if (vw == 0) vw = autotuned_vwave(g, K, motif);
     if (vw != 8 && vw != 16 && vw != 32 && vw != 64)
         return set_error(DM_ERR_ARG, "vwave must be 8, 16, 32 or 64 (got %d)", vw);
-    int lds = cfg.lds_bytes > 0 ? cfg.lds_bytes : tuned_lds(g, K, motif, vw, default_lds(vw));
-    lds = (lds + 15) & ~15;
     int ncls = 0;
     if (motif) ncls = pattern_table(K)->ncls;
+    int lds = cfg.lds_bytes > 0 ? cfg.lds_bytes
+                                : tuned_lds(g, K, motif, vw, max_lds_per_worker(vw, motif ? 1 : 0, K, ncls, 0));
+    lds = (lds + 15) & ~15;
     size_t lds_total = enum_lds_bytes(vw, 256, K, ncls, 0, lds);
     int bpc = 0;
     int r = enum_occupancy(vw, motif ? 1 : 0, 256, lds_total, &bpc);

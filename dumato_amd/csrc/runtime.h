@@ -123,6 +123,7 @@ int run_enum(dm_graph *g, int K, bool motif, const dm_config &cfg, uint64_t *his
              uint64_t *count_out, int sample_mod = 1);
 int tuned_steal_min(const dm_graph *g, int K, bool motif);
 int tuned_lds(const dm_graph *g, int K, bool motif, int vwave, int fallback);
+int max_lds_per_worker(int vwave, int mode, int K, int ncls_single, int hist_slots);
 
 // sched.cpp
 int autotuned_vwave(dm_graph *g, int K, bool motif);

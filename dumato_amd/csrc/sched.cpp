@@ -102,9 +102,11 @@ int autotune(dm_graph *g, int K, bool motif, double budget_ms, dm_config *best_o
             }
         }
     }
-    const int dl = 32768 / (256 / bc.vwave);
+    const int ncls = motif ? pattern_table(K)->ncls : 0;
+    const int dl = max_lds_per_worker(bc.vwave, motif ? 1 : 0, K, ncls, 0);
     if (full_search) {
-        for (int lds : {dl / 2, dl}) {
+        const int d8 = 32768 / (256 / bc.vwave);  // 8 KiB for a full wave
+        for (int lds : {(dl / 2) & ~255, std::min(d8, dl), dl}) {
             for (int sm : {2, 8}) {
                 dm_config c = bc;
                 c.lds_bytes = lds;
