@@ -579,15 +579,31 @@ struct Worker {
             const int tot = g.bcast(sc, take - 1);
             wsync();
             int a = 0;
-            for (int t = g.gl; t < tot; t += W) {
-                while (pre[a + 1] <= t) a++;
-                const int32_t x = p.adj[cs[a] + (t - pre[a])];
-                const int pos = lower_bound(P.uid, pnu, x);
-                if (pos < pnu && P.uid[pos] == x) {
-                    const uint8_t pm = upm_at(P, pos);
-                    if ((pm & 0x80) && x > cc[a]) atomicAdd(&ic[a * npm + (pm & 0x7f)], 1u);
-                } else {
-                    atomicAdd(&nn[a], 1u);
+            for (int t0 = 0; t0 < tot; t0 += W) {
+                const int t = t0 + g.gl;
+                bool fresh = false;  // neighbour outside U: a new element
+                if (t < tot) {
+                    while (pre[a + 1] <= t) a++;
+                    const int32_t x = p.adj[cs[a] + (t - pre[a])];
+                    const int pos = lower_bound(P.uid, pnu, x);
+                    if (pos < pnu && P.uid[pos] == x) {
+                        const uint8_t pm = upm_at(P, pos);
+                        if ((pm & 0x80) && x > cc[a]) atomicAdd(&ic[a * npm + (pm & 0x7f)], 1u);
+                    } else {
+                        fresh = true;
+                    }
+                }
+                // new elements are the common case: count them per child with
+                // one ballot, the child's first lane in this round adding its
+                // segment's popcount (no same-address atomics)
+                const uint64_t mf = g.ballot(fresh);
+                if (t < tot) {
+                    const int s0 = pre[a] > t0 ? pre[a] - t0 : 0;
+                    if (g.gl == s0) {
+                        const int s1 = pre[a + 1] - t0 < W ? pre[a + 1] - t0 : W;
+                        const uint64_t seg = (s1 - s0 >= 64 ? ~0ull : ((1ull << (s1 - s0)) - 1)) << s0;
+                        nn[a] += (uint32_t)__popcll(mf & seg);
+                    }
                 }
             }
             wsync();

@@ -300,31 +300,62 @@ static int train_model(const std::vector<float> &X, const std::vector<float> &Y,
     std::vector<int> idx(n);
     std::iota(idx.begin(), idx.end(), 0);
     Rng rng(seed ^ 0x5bd1e995ull);
-    std::vector<float> bx((size_t)batch * kMlpIn), by(batch);
-    int step = 0;
-    double epoch_loss = 0;
-    for (int ep = 0; ep < epochs; ep++) {
-        for (int i = n - 1; i > 0; i--) std::swap(idx[i], idx[rng.below((uint64_t)i + 1)]);
-        epoch_loss = 0;
-        int nb = 0;
-        for (int s = 0; s + batch <= n || (s < n && nb == 0); s += batch) {
-            const int m = std::min(batch, n - s);
-            for (int t = 0; t < m; t++) {
-                memcpy(&bx[(size_t)t * kMlpIn], &X[(size_t)idx[s + t] * kMlpIn], kMlpIn * 4);
-                by[t] = Y[idx[s + t]];
-            }
-            HIPCHK(hipMemcpy(d.X, bx.data(), (size_t)m * kMlpIn * 4, hipMemcpyHostToDevice));
-            HIPCHK(hipMemcpy(d.Y, by.data(), (size_t)m * 4, hipMemcpyHostToDevice));
-            HIPCHK(hipMemset(d.loss, 0, 4));
-            if ((r = mlp_grad_launch(d.P, d.X, d.Y, m, d.s, d.grad, d.loss, nullptr))) return r;
-            if ((r = mlp_adam_launch(d.P, d.grad, d.M, d.V, 1e-3f, 0.9f, 0.999f, 1e-8f, ++step, nullptr))) return r;
-            float l = 0;
-            HIPCHK(hipMemcpy(&l, d.loss, 4, hipMemcpyDeviceToHost));
-            epoch_loss += l;
-            nb++;
-        }
-        epoch_loss /= std::max(nb, 1);
+    // each epoch's shuffled samples go to the device in one copy; the
+    // minibatches are consecutive slices of it
+    float *dX = nullptr, *dY = nullptr;
+    HIPCHK(hipMalloc((void **)&dX, (size_t)n * kMlpIn * 4));
+    if (hipMalloc((void **)&dY, (size_t)n * 4) != hipSuccess) {
+        (void)hipFree(dX);
+        mlp_dev_free(d);
+        return set_error(DM_ERR_NOMEM, "training samples");
     }
+    std::vector<float> sx((size_t)n * kMlpIn), sy(n);
+    const int nb_epoch = std::max(1, n / batch);
+    // small graphs have few minibatches per epoch: train for at least
+    // kMinSteps Adam steps
+    constexpr int kMinSteps = 1500;
+    const int total_steps = std::max(epochs * nb_epoch, kMinSteps);
+    int step = 0;
+    double last_loss = 0;
+    int last_n = 0;
+    while (step < total_steps && !r) {
+        for (int i = n - 1; i > 0; i--) std::swap(idx[i], idx[rng.below((uint64_t)i + 1)]);
+        for (int t = 0; t < n; t++) {
+            memcpy(&sx[(size_t)t * kMlpIn], &X[(size_t)idx[t] * kMlpIn], kMlpIn * 4);
+            sy[t] = Y[idx[t]];
+        }
+        if (hipMemcpy(dX, sx.data(), sx.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(dY, sy.data(), sy.size() * 4, hipMemcpyHostToDevice) != hipSuccess) {
+            r = set_error(DM_ERR_HIP, "training sample upload");
+            break;
+        }
+        const bool last_epoch = step + nb_epoch >= total_steps;
+        for (int b = 0; b < nb_epoch && step < total_steps; b++) {
+            const int s0 = b * batch, m = std::min(batch, n - s0);
+            if (last_epoch && hipMemset(d.loss, 0, 4) != hipSuccess) {
+                r = set_error(DM_ERR_HIP, "loss reset");
+                break;
+            }
+            if ((r = mlp_grad_launch(d.P, dX + (size_t)s0 * kMlpIn, dY + s0, m, d.s, d.grad, d.loss, nullptr))) break;
+            if ((r = mlp_adam_launch(d.P, d.grad, d.M, d.V, 1e-3f, 0.9f, 0.999f, 1e-8f, ++step, nullptr))) break;
+            if (last_epoch) {
+                float l = 0;
+                if (hipMemcpy(&l, d.loss, 4, hipMemcpyDeviceToHost) != hipSuccess) {
+                    r = set_error(DM_ERR_HIP, "loss readback");
+                    break;
+                }
+                last_loss += l;
+                last_n++;
+            }
+        }
+    }
+    (void)hipFree(dX);
+    (void)hipFree(dY);
+    if (r) {
+        mlp_dev_free(d);
+        return r;
+    }
+    const double epoch_loss = last_loss / std::max(last_n, 1);
     HIPCHK(hipMemcpy(P.data(), d.P, kMlpParams * 4, hipMemcpyDeviceToHost));
     mlp_dev_free(d);
     final_loss = epoch_loss;
@@ -451,20 +482,27 @@ int sched_train(dm_graph *g, int mode_motif, int k_train, int epochs, uint64_t s
     const bool motif = mode_motif != 0;
     std::vector<uint64_t> hist;
     uint64_t tot = 0;
-    if (motif) {
-        const PatternTable *t = pattern_table(k_train);
-        if (!t) return last_code();
-        hist.assign(t->ncls, 0);
-        r = run_enum(g, k_train, true, cfg, hist.data(), hist.size(), &tot);
-    } else {
-        r = run_enum(g, k_train, false, cfg, nullptr, 0, &tot);
+    // per-root cycles summed over a few runs: a single run's small roots
+    // are dominated by timing noise
+    constexpr int kRuns = 3;
+    std::vector<double> cost(g->rcsr.n, 0.0);
+    for (int run = 0; run < kRuns; run++) {
+        if (motif) {
+            const PatternTable *t = pattern_table(k_train);
+            if (!t) return last_code();
+            hist.assign(t->ncls, 0);
+            r = run_enum(g, k_train, true, cfg, hist.data(), hist.size(), &tot);
+        } else {
+            r = run_enum(g, k_train, false, cfg, nullptr, 0, &tot);
+        }
+        if (r) return r;
+        for (size_t v = 0; v < cost.size(); v++) cost[v] += (double)g->last_costs[v];
     }
-    if (r) return r;
     // samples: roots with work, target log(cycles)
     const int32_t n = g->rcsr.n;
     std::vector<int32_t> roots;
     for (int32_t v = 0; v < n; v++)
-        if (g->last_costs[g->inv[v]] > 0) roots.push_back(v);
+        if (cost[g->inv[v]] > 0) roots.push_back(v);
     if (roots.size() < 16) return set_error(DM_ERR_ARG, "too few roots with work to train on");
     CostModel *M = new CostModel();
     M->mode_motif = mode_motif;
@@ -484,7 +522,7 @@ int sched_train(dm_graph *g, int mode_motif, int k_train, int epochs, uint64_t s
     }
     double ys = 0, ys2 = 0;
     for (size_t t = 0; t < ns; t++) {
-        double y = std::log((double)g->last_costs[g->inv[roots[t]]]);
+        double y = std::log(cost[g->inv[roots[t]]] / kRuns);
         Y[t] = (float)y;
         ys += y;
         ys2 += y * y;
