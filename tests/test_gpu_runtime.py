@@ -1,0 +1,95 @@
+"""Runtime mechanics on the GPU: the device timeline in dm_stats, the
+donation-check period, the worker stacks spilling to the global arena, the
+static-partition cache, and heavy donation traffic.  Every case must keep
+the counts exact (against the oracle or the committed full-size goldens)."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import dumato_amd as dm
+from dumato_amd import workloads as W
+from oracle import pyoracle as O
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def cfg(**kw):
+    kw.setdefault("watchdog_s", 60.0)
+    return dm.config(**kw)
+
+
+@pytest.fixture(scope="module")
+def citeseer():
+    path = os.path.join(GOLD, "datasets", "citeseer.edgelist")
+    return dm.Graph.from_edgelist(path), O.load_edgelist(path)
+
+
+@pytest.fixture(scope="module")
+def rmat():
+    return W.rmat_s17()
+
+
+@pytest.fixture(scope="module")
+def golden():
+    with open(os.path.join(GOLD, "workload_counts.json")) as f:
+        return json.load(f)
+
+
+def test_timeline_is_ordered(rmat):
+    print("case timeline rmat k=4", flush=True)
+    rmat.clique_count(4, cfg())
+    st = rmat.last_stats()
+    assert 0 < st["first_job_ms"] <= st["last_job_ms"] <= st["exit_ms"]
+    assert st["static_done_ms"] <= st["exit_ms"]
+    # the exit of the last block is inside the events bracketing the launch
+    assert st["exit_ms"] <= st["device_ms"] * 1.05 + 0.05
+    assert st["jobs"] >= 1 and st["busy_max_cycles"] > 0
+
+
+@pytest.mark.parametrize("period", [0, 2, 50])
+def test_donation_check_period_keeps_counts(citeseer, period):
+    g, ref = citeseer
+    print("case steal_period=%d" % period, flush=True)
+    assert g.clique_count(4, cfg(steal_period=period, steal_min=2)) == O.clique_count(ref, 4)
+    tot, hist = g.motif_count(4, cfg(steal_period=period, steal_min=2))
+    wt, wh = O.motif_count(ref, 4)
+    assert tot == wt and np.array_equal(hist, wh)
+
+
+@pytest.mark.parametrize("lds", [256, 1024])
+def test_small_lds_stacks_spill_to_the_arena(rmat, golden, lds):
+    # tiny LDS slices put the level lists, leaf scratch and root bitmaps in
+    # the global arena; counts must not change
+    want = golden["rmat_s17"]["clique"]
+    for k in (4, 5):
+        print("case lds=%d clique k=%d" % (lds, k), flush=True)
+        assert rmat.clique_count(k, cfg(lds_bytes=lds, vwave=64, steal_min=2)) == want[str(k)]
+
+
+def test_heavy_donation_traffic_keeps_counts(rmat, golden):
+    # donate as often and as finely as the runtime allows
+    want = golden["rmat_s17"]["clique"]
+    print("case heavy donation k=6", flush=True)
+    assert rmat.clique_count(6, cfg(steal_min=2, steal_period=0, vwave=64)) == want["6"]
+    assert rmat.last_stats()["donations"] > 0
+
+
+def test_partition_cache_follows_root_range_and_model(citeseer):
+    g, ref = citeseer
+    n = g.info()[0]
+    lo, hi = n // 4, n // 2
+    print("case partition cache", flush=True)
+    part = g.clique_count(4, cfg(root_begin=lo, root_end=hi))
+    full = g.clique_count(4, cfg())
+    assert full == O.clique_count(ref, 4)
+    assert g.clique_count(4, cfg(root_begin=lo, root_end=hi)) == part
+    # learned partitions are rebuilt after each training
+    g.sched_train(False, 3, epochs=5, seed=1)
+    a = g.clique_count(4, cfg(sched=dm.DM_SCHED_LEARNED))
+    g.sched_train(False, 3, epochs=5, seed=2)
+    b = g.clique_count(4, cfg(sched=dm.DM_SCHED_LEARNED))
+    assert a == b == full

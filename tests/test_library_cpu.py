@@ -34,6 +34,16 @@ def test_config_defaults():
     c = dm.config()
     assert c.sched == dm.DM_SCHED_PLAIN and c.steal == 1 and c.steal_min == 0
     assert c.vwave == 0 and c.watchdog_s > 0
+    assert c.steal_period == 2  # microseconds between donation checks
+
+
+@pytest.mark.parametrize("bad", [dict(steal_min=1), dict(steal_min=-3), dict(steal_period=-1)])
+def test_config_validation_before_any_device_work(bad):
+    # rejected on the host, before the device is touched
+    g = dm.Graph.erdos_renyi(50, 100, 1)
+    with pytest.raises(dm.DMError) as e:
+        g.clique_count(3, dm.config(**bad))
+    assert "steal_min" in str(e.value)
 
 
 @pytest.mark.parametrize("k", [3, 4, 5, 6])
