@@ -108,7 +108,39 @@ def query_block(graph, mode, k, sched, warmup, steps):
             "subgraphs_per_s": count / ((km + sm) / 1e3), "wall_ms_per_step": 1e3 * wall / steps,
             "vwave": st["vwave"], "workers": st["workers"], "donations": st["donations"],
             "balance": st["busy_sum_cycles"] / max(1, st["workers"] * st["busy_max_cycles"]),
+            "busy_max_mcycles": st["busy_max_cycles"] / 1e6,
             "alg_GBs": st["alg_bytes"] / (km / 1e3) / 1e9}
+
+
+def rank_corr(a, b):
+    """Spearman rank correlation (no tie correction)."""
+    import numpy as np
+    ra = np.argsort(np.argsort(a)).astype(np.float64)
+    rb = np.argsort(np.argsort(b)).astype(np.float64)
+    return float(np.corrcoef(ra, rb)[0, 1])
+
+
+def model_quality(graph, mode, k):
+    """Predicted (cost model) vs measured per-root cycles of a plain run
+    without stealing at size k: Spearman of the ranks and MAPE of the cost
+    (the model predicts log cycles of its training size, so MAPE is only
+    meaningful at that size)."""
+    import numpy as np
+
+    import dumato_amd as dm
+    pred = np.asarray(graph.sched_predict(), np.float64)
+    cfg = dm.config(steal=0, record_costs=1)
+    if mode == "motif":
+        graph.motif_count(k, cfg)
+    else:
+        graph.clique_count(k, cfg)
+    cost = graph.last_root_costs().astype(np.float64)
+    has = cost > 0
+    est = np.exp(pred[has])
+    heavy = cost >= np.quantile(cost[has], 0.9)
+    return {"k": k, "roots": int(has.sum()), "spearman": rank_corr(pred[has], cost[has]),
+            "spearman_top10pct": rank_corr(pred[heavy], cost[heavy]),
+            "mape": float(np.mean(np.abs(est - cost[has]) / cost[has]))}
 
 
 def plain_vs_learned(graph, mode, k_list, k_train, warmup, steps, budget_ms=300.0):
@@ -118,6 +150,8 @@ def plain_vs_learned(graph, mode, k_list, k_train, warmup, steps, budget_ms=300.
     loss = graph.sched_train(mode == "motif", k_train, epochs=30, seed=1)
     out["train"] = {"k_train": k_train, "loss": loss, "seconds": time.perf_counter() - t}
     progress("%s: cost model trained on k=%d (loss %.3f)" % (mode, k_train, loss))
+    out["model_vs_measured"] = [model_quality(graph, mode, k) for k in (k_train,) + tuple(k_list)]
+    progress("%s: model quality measured" % mode)
     for k in k_list:
         graph.autotune(mode == "motif", k, budget_ms)
         progress("%s k=%d: autotuned" % (mode, k))
@@ -128,7 +162,8 @@ def plain_vs_learned(graph, mode, k_list, k_train, warmup, steps, budget_ms=300.
         assert p["subgraphs"] == lq["subgraphs"], "learned and plain counts differ"
         out["k%d" % k] = {"plain": p, "learned": lq,
                           "learned_speedup": p["query_ms"] / lq["query_ms"],
-                          "learned_kernel_speedup": p["kernel_ms"] / lq["kernel_ms"]}
+                          "learned_kernel_speedup": p["kernel_ms"] / lq["kernel_ms"],
+                          "makespan_reduction": 1.0 - lq["busy_max_mcycles"] / p["busy_max_mcycles"]}
     return out
 
 

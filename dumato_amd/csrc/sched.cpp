@@ -26,11 +26,6 @@ namespace dm {
         if (e_ != hipSuccess) return set_error(DM_ERR_HIP, "%s: %s", #x, hipGetErrorString(e_));     \
     } while (0)
 
-static double now_ms() {
-    using namespace std::chrono;
-    return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
-}
-
 // Width used when no tuned value is cached for (mode, k): full 64-lane
 // waves, the fastest untuned choice on every benchmark graph measured.
 int autotuned_vwave(dm_graph *g, int K, bool motif) {
@@ -573,6 +568,15 @@ extern "C" int dm_sched_predict(dm_graph *g, float *pred, size_t n) {
     return DM_OK;
 }
 
+// Runs a HIP call unless an earlier one failed; records the first failure.
+#define MLP_TRY(call, what)                                                                          \
+    do {                                                                                             \
+        if (!r) {                                                                                    \
+            hipError_t e_ = (call);                                                                  \
+            if (e_ != hipSuccess) r = set_error(DM_ERR_HIP, "%s: %s", what, hipGetErrorString(e_));    \
+        }                                                                                            \
+    } while (0)
+
 // Test entry points for the MLP kernels (no graph involved).
 extern "C" int dm_mlp_grad(const float *params, const float *X, const float *Y, int n, float *out, float *grad,
                            float *loss) {
@@ -582,17 +586,15 @@ extern "C" int dm_mlp_grad(const float *params, const float *X, const float *Y, 
     MlpDev d;
     int r = mlp_dev_alloc(d, n);
     if (r) return r;
-    hipMemcpy(d.P, params, kMlpParams * 4, hipMemcpyHostToDevice);
-    hipMemcpy(d.X, X, (size_t)n * kMlpIn * 4, hipMemcpyHostToDevice);
-    hipMemcpy(d.Y, Y, (size_t)n * 4, hipMemcpyHostToDevice);
-    hipMemset(d.loss, 0, 4);
-    r = mlp_grad_launch(d.P, d.X, d.Y, n, d.s, d.grad, d.loss, nullptr);
-    if (!r) {
-        hipMemcpy(grad, d.grad, kMlpParams * 4, hipMemcpyDeviceToHost);
-        hipMemcpy(loss, d.loss, 4, hipMemcpyDeviceToHost);
-        if (out) hipMemcpy(out, d.s.out, (size_t)n * 4, hipMemcpyDeviceToHost);
-        if (hipDeviceSynchronize() != hipSuccess) r = set_error(DM_ERR_HIP, "MLP gradient");
-    }
+    MLP_TRY(hipMemcpy(d.P, params, kMlpParams * 4, hipMemcpyHostToDevice), "parameter upload");
+    MLP_TRY(hipMemcpy(d.X, X, (size_t)n * kMlpIn * 4, hipMemcpyHostToDevice), "sample upload");
+    MLP_TRY(hipMemcpy(d.Y, Y, (size_t)n * 4, hipMemcpyHostToDevice), "target upload");
+    MLP_TRY(hipMemset(d.loss, 0, 4), "loss reset");
+    if (!r) r = mlp_grad_launch(d.P, d.X, d.Y, n, d.s, d.grad, d.loss, nullptr);
+    MLP_TRY(hipMemcpy(grad, d.grad, kMlpParams * 4, hipMemcpyDeviceToHost), "gradient readback");
+    MLP_TRY(hipMemcpy(loss, d.loss, 4, hipMemcpyDeviceToHost), "loss readback");
+    if (out) MLP_TRY(hipMemcpy(out, d.s.out, (size_t)n * 4, hipMemcpyDeviceToHost), "output readback");
+    MLP_TRY(hipDeviceSynchronize(), "MLP gradient");
     mlp_dev_free(d);
     return r;
 }
@@ -605,21 +607,19 @@ extern "C" int dm_mlp_adam(float *params, const float *X, const float *Y, int n,
     MlpDev d;
     int r = mlp_dev_alloc(d, n);
     if (r) return r;
-    hipMemcpy(d.P, params, kMlpParams * 4, hipMemcpyHostToDevice);
-    hipMemset(d.M, 0, kMlpParams * 4);
-    hipMemset(d.V, 0, kMlpParams * 4);
-    hipMemcpy(d.X, X, (size_t)n * kMlpIn * 4, hipMemcpyHostToDevice);
-    hipMemcpy(d.Y, Y, (size_t)n * 4, hipMemcpyHostToDevice);
+    MLP_TRY(hipMemcpy(d.P, params, kMlpParams * 4, hipMemcpyHostToDevice), "parameter upload");
+    MLP_TRY(hipMemset(d.M, 0, kMlpParams * 4), "moment reset");
+    MLP_TRY(hipMemset(d.V, 0, kMlpParams * 4), "moment reset");
+    MLP_TRY(hipMemcpy(d.X, X, (size_t)n * kMlpIn * 4, hipMemcpyHostToDevice), "sample upload");
+    MLP_TRY(hipMemcpy(d.Y, Y, (size_t)n * 4, hipMemcpyHostToDevice), "target upload");
     for (int s = 0; s < steps && !r; s++) {
-        hipMemset(d.loss, 0, 4);
-        r = mlp_grad_launch(d.P, d.X, d.Y, n, d.s, d.grad, d.loss, nullptr);
+        MLP_TRY(hipMemset(d.loss, 0, 4), "loss reset");
+        if (!r) r = mlp_grad_launch(d.P, d.X, d.Y, n, d.s, d.grad, d.loss, nullptr);
         if (!r) r = mlp_adam_launch(d.P, d.grad, d.M, d.V, lr, 0.9f, 0.999f, 1e-8f, s + 1, nullptr);
-        if (!r && losses) hipMemcpy(&losses[s], d.loss, 4, hipMemcpyDeviceToHost);
+        if (losses) MLP_TRY(hipMemcpy(&losses[s], d.loss, 4, hipMemcpyDeviceToHost), "loss readback");
     }
-    if (!r) {
-        hipMemcpy(params, d.P, kMlpParams * 4, hipMemcpyDeviceToHost);
-        if (hipDeviceSynchronize() != hipSuccess) r = set_error(DM_ERR_HIP, "MLP Adam");
-    }
+    MLP_TRY(hipMemcpy(params, d.P, kMlpParams * 4, hipMemcpyDeviceToHost), "parameter readback");
+    MLP_TRY(hipDeviceSynchronize(), "MLP Adam");
     mlp_dev_free(d);
     return r;
 }
