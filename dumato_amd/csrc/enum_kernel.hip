@@ -558,7 +558,9 @@ struct Worker {
         int i = hi - 1;
         while (i >= lo) {
             if (check_abort()) return;
-            maybe_donate(l, l, lo, i);
+            // leaf children are cheap and a donated piece rebuilds its whole
+            // prefix: offer them only from long leaf loops
+            if (i - lo + 1 >= 512) maybe_donate(l, l, lo, i);
             if (i < lo) break;
             const int avail = i - lo + 1 < B ? i - lo + 1 : B;
             int len = 0, pm_a = -1;
