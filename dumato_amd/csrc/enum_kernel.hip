@@ -551,6 +551,24 @@ struct Worker {
         for (int j = hi + g.gl; j < P.ne; j += W) atomicAdd(&ws.n_gt()[epm_at(P, j)], 1u);
         wsync();
         const int pnu = P.nu;
+        // Id directory of U: ids above the root split into at most 256
+        // buckets of 2^dsh ids; dir[b] = first position of bucket b.  A lookup
+        // computes its bucket and binary-searches only that bucket.
+        int32_t *dir = nullptr;
+        int dsh = 0, dnb = 0;
+        const int32_t dbase = v0 + 1;
+        if (pnu >= 32) {
+            const int32_t span = P.uid[pnu - 1] - dbase + 1;
+            while ((span >> dsh) > 256) dsh++;
+            dnb = ((span - 1) >> dsh) + 1;
+            const int64_t need = (int64_t)(dnb + 1) * 4;
+            if (lds_top + ((need + 15) & ~(int64_t)15) <= lds_lo + p.lds_per_worker) {
+                dir = (int32_t *)alloc(need);
+                for (int b = g.gl; b <= dnb; b += W)
+                    dir[b] = b == dnb ? pnu : lower_bound(P.uid, pnu, dbase + (b << dsh));
+                wsync();
+            }
+        }
         // a batch takes children until their neighbour lists hold about
         // `target` elements: few, large batches amortise the per-batch
         // class bookkeeping
@@ -587,7 +605,16 @@ struct Worker {
                 if (t < tot) {
                     while (pre[a + 1] <= t) a++;
                     const int32_t x = p.adj[cs[a] + (t - pre[a])];
-                    const int pos = lower_bound(P.uid, pnu, x);
+                    int pos = pnu;
+                    if (!dir) {
+                        pos = lower_bound(P.uid, pnu, x);
+                    } else if (x >= dbase) {
+                        const int b = (x - dbase) >> dsh;
+                        if (b < dnb) {
+                            const int lo = dir[b];
+                            pos = lo + lower_bound(P.uid + lo, dir[b + 1] - lo, x);
+                        }
+                    }
                     if (pos < pnu && P.uid[pos] == x) {
                         const uint8_t pm = upm_at(P, pos);
                         if ((pm & 0x80) && x > cc[a]) atomicAdd(&ic[a * npm + (pm & 0x7f)], 1u);
