@@ -46,7 +46,8 @@ class Stats(ctypes.Structure):
                 ("busy_max_cycles", ctypes.c_uint64), ("vwave", ctypes.c_int32), ("workers", ctypes.c_int32),
                 ("blocks", ctypes.c_int32), ("lds_bytes", ctypes.c_int32),
                 ("first_job_ms", ctypes.c_double), ("static_done_ms", ctypes.c_double),
-                ("last_job_ms", ctypes.c_double), ("exit_ms", ctypes.c_double)]
+                ("last_job_ms", ctypes.c_double), ("exit_ms", ctypes.c_double),
+                ("job_max_cycles", ctypes.c_uint64)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}

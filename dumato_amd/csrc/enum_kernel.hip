@@ -144,8 +144,8 @@ struct Lvl {
 // holds a single pointer to it: uint64 st[kStats] (the worker's statistics),
 // int32 jid[kJobIds] (the prefix of the current job), uint32 n_gt[kMaxPm]
 // (motif leaf-phase counters), then Lvl lv[nlv], nlv = max(K-2, 1).
-enum { kStBusy, kStJobs, kStFirst, kStStatic, kStLast, kStQCount, kStDonated, kStT0, kStats };
-constexpr int kJidOff = 64;
+enum { kStBusy, kStJobs, kStFirst, kStStatic, kStLast, kStQCount, kStDonated, kStT0, kStJobMax, kStats };
+constexpr int kJidOff = 80;
 constexpr int kNgtOff = kJidOff + kJobIds * 4;
 constexpr int kLvOff = (kNgtOff + kMaxPm * 4 + 15) & ~15;
 static_assert(kStats * 8 <= kJidOff, "worker statistics overlap the job prefix");
@@ -369,14 +369,16 @@ struct Worker {
     // leaf_k / leaf_lo / leaf_i describe the descending leaf-phase loop of
     // level leaf_k (remaining children [lo, i]); leaf_k < 0 when not in it.
     static constexpr int kMaxPieces = 8;
-    DEV void maybe_donate(int top, int leaf_k, int &leaf_lo, int leaf_i) {
-        if (!p.steal) return;
+    // Waiting tickets (at most kMaxPieces), checked at most every poll
+    // period; 0 when stealing is off.  A worker takes a ticket once no
+    // static job is left.  The check is a hint: a workgroup-scope load may be
+    // served from this CU's cache and lag by a moment, which only delays a
+    // donation; the push is exact.
+    DEV int donation_demand() {
+        if (!p.steal) return 0;
         const uint64_t now = now_ticks();
-        if (now - last_poll < p.poll_ticks) return;
+        if (now - last_poll < p.poll_ticks) return 0;
         last_poll = now;
-        // a worker takes a ticket once no static job is left.  The check is a
-        // hint: a workgroup-scope load may be served from this CU's cache and
-        // lag by a moment, which only delays a donation; the push is exact.
         int want = 0;
         if (g.gl == 0) {
             const unsigned long long ht =
@@ -384,7 +386,11 @@ struct Worker {
             const uint32_t h = (uint32_t)(ht >> 32), t = (uint32_t)ht;
             want = h > t ? (int)(h - t < (uint32_t)kMaxPieces ? h - t : (uint32_t)kMaxPieces) : 0;
         }
-        want = g.bcast(want, 0);
+        return g.bcast(want, 0);
+    }
+
+    DEV void maybe_donate(int top, int leaf_k, int &leaf_lo, int leaf_i) {
+        const int want = donation_demand();
         if (want == 0) return;
         // untouched roots of a range job go first: they need no prefix
         if (range_hi - v0 - 1 >= 2) {
@@ -1343,6 +1349,7 @@ __global__ __launch_bounds__(kThreads, (min_waves<W, MODE>())) void enum_kernel(
                         const unsigned long long dt = __builtin_amdgcn_s_memtime() - ws.st()[kStT0];
                         ws.st()[kStQCount] += jc;
                         ws.st()[kStBusy] += dt;
+                        if (dt > ws.st()[kStJobMax]) ws.st()[kStJobMax] = dt;
                         ws.st()[kStLast] = wall_clock64();
                         if (p.root_cost) atomicAdd(&p.root_cost[root], dt);
                         if (p.root_count && jc) atomicAdd(&p.root_count[root], jc);
@@ -1364,7 +1371,7 @@ __global__ __launch_bounds__(kThreads, (min_waves<W, MODE>())) void enum_kernel(
     }
     // Per-worker statistics (held by lane 0 of each worker) reduced over the
     // block, so that each block, not each worker, adds to the shared words.
-    constexpr int NF = 9;
+    constexpr int NF = 10;
     __shared__ unsigned long long red[kThreads / 64][NF];
     const bool holder = wid < p.nworkers && g.gl == 0;
     const unsigned long long *st = ws.st();
@@ -1372,7 +1379,8 @@ __global__ __launch_bounds__(kThreads, (min_waves<W, MODE>())) void enum_kernel(
                                 holder ? st[kStBusy] : 0ull,    holder ? st[kStDonated] : 0ull,
                                 holder && single ? st[kStQCount] : 0ull,
                                 holder ? st[kStBusy] : 0ull,    holder ? st[kStFirst] : 0ull,
-                                holder ? st[kStStatic] : 0ull,  holder ? st[kStLast] : 0ull};
+                                holder ? st[kStStatic] : 0ull,  holder ? st[kStLast] : 0ull,
+                                holder ? st[kStJobMax] : 0ull};
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
 #pragma unroll
@@ -1399,6 +1407,7 @@ __global__ __launch_bounds__(kThreads, (min_waves<W, MODE>())) void enum_kernel(
         atomicMax(&p.sc->t_first_job, f[6]);
         atomicMax(&p.sc->t_static_done, f[7]);
         atomicMax(&p.sc->t_last_job, f[8]);
+        atomicMax(&p.sc->job_max, f[9]);
         atomicMax(&p.sc->t_exit, (unsigned long long)wall_clock64());
     }
 }
@@ -1433,6 +1442,7 @@ __global__ void enum_init_kernel(SchedCounters *sc, unsigned long long njobs, in
         sc->t_static_done = 0;
         sc->t_last_job = 0;
         sc->t_exit = 0;
+        sc->job_max = 0;
         *epoch = *epoch + 1;
         if (start_ticks) *start_ticks = ~0ull;
     }

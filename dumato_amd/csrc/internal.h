@@ -102,6 +102,7 @@ struct SchedCounters {
     // to start its first job, to find static work exhausted, to end its last
     // job and to exit
     unsigned long long t_start, t_first_job, t_static_done, t_last_job, t_exit;
+    unsigned long long job_max;  // cycles of the longest single job
 };
 
 // One counting query of a launch.  A single-graph call has one; a batch

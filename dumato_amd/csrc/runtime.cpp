@@ -500,6 +500,7 @@ int run_enum(dm_graph *g, int K, bool motif, const dm_config &cfg, uint64_t *his
         st.static_done_ms = rel(sc1.t_static_done);
         st.last_job_ms = rel(sc1.t_last_job);
         st.exit_ms = rel(sc1.t_exit);
+        st.job_max_cycles = sc1.job_max;
     }
     st.vwave = sh.vwave;
     st.workers = sh.workers;

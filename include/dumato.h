@@ -74,6 +74,8 @@ typedef struct dm_stats {
      * its first job, to find the static queues empty, to end its last job,
      * and to exit */
     double first_job_ms, static_done_ms, last_job_ms, exit_ms;
+    uint64_t job_max_cycles;  /* cycles of the longest single job (a lower bound
+                                 on the makespan that stealing cannot cut) */
 } dm_stats;
 
 /* Fills *cfg with the defaults (plain scheduler, autotuned width/LDS). */
