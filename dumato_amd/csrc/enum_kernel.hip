@@ -193,6 +193,7 @@ struct Worker {
     int range_hi;  // end of the current range job's roots (v0 + 1 otherwise)
     uint64_t t_start;
     uint64_t last_poll;  // wall clock of the last donation check
+    uint32_t polls;      // donation checks so far (lane 0's count)
     bool abort;
     // motif leaf-phase scratch, reserved first in the LDS slice of each job:
     // cs int64[B] | ic u32[B][npm] | nn u32[B] | cc i32[B] | pre i32[B + 1]
@@ -225,6 +226,7 @@ struct Worker {
         abort = false;
         t_start = now_ticks();
         last_poll = t_start;
+        polls = 0;
     }
 
     // ---- helpers -----------------------------------------------------------
@@ -371,9 +373,9 @@ struct Worker {
     static constexpr int kMaxPieces = 8;
     // Waiting tickets (at most kMaxPieces), checked at most every poll
     // period; 0 when stealing is off.  A worker takes a ticket once no
-    // static job is left.  The check is a hint: a workgroup-scope load may be
-    // served from this CU's cache and lag by a moment, which only delays a
-    // donation; the push is exact.
+    // static job is left.  The check is a hint: most checks are a
+    // workgroup-scope load that may be served from this CU's cache and lag,
+    // which only delays a donation; the push itself is exact.
     DEV int donation_demand() {
         if (!p.steal) return 0;
         const uint64_t now = now_ticks();
@@ -381,15 +383,21 @@ struct Worker {
         last_poll = now;
         int want = 0;
         if (g.gl == 0) {
+            // every 4th check reads through to the device scope: a worker
+            // whose loop touches little global memory keeps the line in its
+            // CU's cache, and a cached copy could hide demand indefinitely
             const unsigned long long ht =
-                __hip_atomic_load(&p.sc->q_ht, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                (++polls & 3) == 0 ? __hip_atomic_load(&p.sc->q_ht, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                                   : __hip_atomic_load(&p.sc->q_ht, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
             const uint32_t h = (uint32_t)(ht >> 32), t = (uint32_t)ht;
             want = h > t ? (int)(h - t < (uint32_t)kMaxPieces ? h - t : (uint32_t)kMaxPieces) : 0;
         }
         return g.bcast(want, 0);
     }
 
-    DEV void maybe_donate(int top, int leaf_k, int &leaf_lo, int leaf_i) {
+    // min_first > 0: the job's own level may be cut down to pieces of that
+    // many children (heavy children, where steal_min would leave too much)
+    DEV void maybe_donate(int top, int leaf_k, int &leaf_lo, int leaf_i, int min_first = 0) {
         const int want = donation_demand();
         if (want == 0) return;
         // untouched roots of a range job go first: they need no prefix
@@ -402,7 +410,7 @@ struct Worker {
         const int leaf_min = 4 * p.steal_min;
         for (int l = job_lvl; l < top; l++) {
             const int cur = ws.lv()[l].cur, end = ws.lv()[l].end;
-            const int minc = l == K - 3 ? leaf_min : p.steal_min;
+            const int minc = (min_first > 0 && l == job_lvl) ? min_first : l == K - 3 ? leaf_min : p.steal_min;
             if (end - cur >= minc) {
                 int P = (end - cur) / minc - 1;
                 if (P < 1) P = 1;
@@ -930,7 +938,15 @@ struct Worker {
         while (true) {
             if (check_abort()) break;
             int dummy = 0;
-            maybe_donate(1, -1, dummy, 0);
+            // at k >= 6 a front child with many pairs (its work grows with
+            // their cube) is heavy enough to travel alone
+            int front = 0;
+            if (K >= 6 && g.gl == 0 && R.cur < R.end) {
+#pragma unroll
+                for (int w = 0; w < NW; w++) front += __popcll(rows[R.cur * NW + w]);
+            }
+            front = g.bcast(front, 0);
+            maybe_donate(1, -1, dummy, 0, front >= W / 2 ? 2 : 0);
             const int cur = R.cur, end = R.end;
             if (cur >= end) break;
             // a batch holds children until it has about `target` pairs (at
