@@ -166,15 +166,15 @@ static int batch_build(dm_batch *b, const std::vector<dm_query> &qs, const dm_co
     // launch shape: width / LDS from the config (autotuned values cached in the plan)
     int vw = cfg.vwave ? cfg.vwave : (b->plan.vwave ? b->plan.vwave : 64);
     if (vw != 8 && vw != 16 && vw != 32 && vw != 64) return set_error(DM_ERR_ARG, "vwave must be 8/16/32/64");
-    int lds = cfg.lds_bytes > 0 ? cfg.lds_bytes : 32768 / (256 / vw);
-    lds = (lds + 15) & ~15;
+    // 0: each phase takes the largest LDS stack its occupancy allows
+    const int lds = cfg.lds_bytes > 0 ? (cfg.lds_bytes + 15) & ~15 : 0;
     const int sm = cfg.steal_min ? cfg.steal_min : (b->tuned_steal_min ? b->tuned_steal_min : 2);
     if (b->exec && b->plan.same(qs, cfg, vw, lds, sm)) return DM_OK;
     drop_graph(b);
     // phases: (mode, kmax, per-worker class slots) per family present; a
     // single query keeps the batch's per-worker counters (mode 2)
     struct Phase {
-        int mode, kmax, wslots, blocks, workers;
+        int mode, kmax, wslots, blocks, workers, lds;
         size_t lds_total;
         std::vector<int64_t> sq_off;
         std::vector<int32_t> sq_jobs;
@@ -202,7 +202,9 @@ static int batch_build(dm_batch *b, const std::vector<dm_query> &qs, const dm_co
         // one query: the kernel keeps block-wide class counters (sized by
         // the query's classes) instead of per-worker ones
         const int ncls_single = qs.size() == 1 ? P.wslots : 0;
-        P.lds_total = enum_lds_bytes(vw, 256, P.kmax, ncls_single, qs.size() == 1 ? 0 : P.wslots, lds);
+        const int hs = qs.size() == 1 ? 0 : P.wslots;
+        P.lds = lds > 0 ? lds : max_lds_per_worker(vw, P.mode, P.kmax, ncls_single, hs);
+        P.lds_total = enum_lds_bytes(vw, 256, P.kmax, ncls_single, hs, P.lds);
         int bpc = 0;
         if ((r = enum_occupancy(vw, P.mode, 256, P.lds_total, &bpc))) return r;
         if (bpc < 1) return set_error(DM_ERR_ARG, "batch launch does not fit one CU (LDS %zu B)", P.lds_total);
@@ -285,7 +287,7 @@ static int batch_build(dm_batch *b, const std::vector<dm_query> &qs, const dm_co
         p.sc = b->d_sc + pi;
         p.arena = d->d_arena;
         p.arena_per_worker = arena;
-        p.lds_per_worker = lds;
+        p.lds_per_worker = P.lds;
         p.steal = cfg.steal ? 1 : 0;
         p.steal_min = sm;
         p.poll_ticks = (uint64_t)(cfg.steal_period >= 0 ? cfg.steal_period : 2) * (uint64_t)d->wall_khz / 1000;
@@ -327,7 +329,7 @@ static int batch_build(dm_batch *b, const std::vector<dm_query> &qs, const dm_co
     b->last.vwave = vw;
     b->last.workers = max_workers;
     b->last.blocks = ph.back().blocks;
-    b->last.lds_bytes = lds;
+    b->last.lds_bytes = ph.back().lds;
     return DM_OK;
 }
 
@@ -534,7 +536,7 @@ int dm_batch_autotune(dm_batch *b, const dm_query *queries, size_t nq, double bu
             }
         }
     }
-    bc.lds_bytes = 32768 / (256 / bc.vwave);
+    bc.lds_bytes = 0;  // each phase: the largest stack its occupancy allows
     b->plan.vwave = bc.vwave;
     b->tuned_steal_min = bc.steal_min;
     if (best) *best = bc;
