@@ -7,7 +7,9 @@ Headline (value): k=4 motif histogram of the seeded Chung-Lu graph
 Same JSON line, extra fields:
   clique_rmat17   k=5 and k=6 clique counts of the seeded R-MAT scale-17
                   graph, plain vs learned scheduler (the cost model is trained
-                  on per-root cycle counts of a k=4 run of the same graph);
+                  on per-root cycle counts of k=4 runs of the same graph; its
+                  Spearman/MAPE against measured per-root cycles at k=4..6 and
+                  the makespan reduction are reported);
   learned_speedup plain / learned time per query, where time = enumeration
                   kernel + host scheduling (inference, ordering, binning);
   motif5_ba       (--full) k=5 motif histogram of the Barabasi-Albert graph
@@ -18,8 +20,12 @@ result copy); the graph is resident on the GPU before timing starts and the
 launch parameters are autotuned once before timing.  value = subgraphs per
 second of wall time over the timed steps.  The roofline uses the enumeration
 kernel's own duration (HIP events on the library's stream, median over the
-steps) and its algorithmic bytes (the measurement plan's formula, summed on
-the device and checked against the CPU oracle in tests).
+steps) and its algorithmic bytes: 4 bytes per adjacency entry the DFS reads
+(each member list of every node, plus k-1 adjacency tests per motif leaf;
+for cliques the out-lists of the candidates of nodes with 1..k-2 vertices),
+summed on the device and checked against the CPU oracle in tests.  The
+traffic field is the measured HBM bytes per launch of the same kernel
+(rocprofv3 FETCH_SIZE and WRITE_SIZE, calibrated on a copy kernel).
 
 The CPU baseline is the oracle's port of the reference's DFS logic
 (clique_DM_DFS / motifs_DM_DFS: thread per root, linear-scan adjacency), with
