@@ -87,3 +87,31 @@ def test_autotune_keeps_counts(graph):
     tot, hist = g.motif_count(4)
     wt, wh = O.motif_count(ref, 4)
     assert tot == wt and np.array_equal(hist, wh) and g.last_stats()["vwave"] == bm.vwave
+
+
+def test_learned_gain_over_plain_is_measured(graph):
+    # What the trained network adds over the method without it (the plain
+    # degree order): its ranking of per-root costs against the out-degree
+    # ranking, and the kernel time of the learned partition against plain.
+    # Counts must match; the gain is reported, not assumed (on these graphs
+    # the on-device donation already balances the plain partition).
+    g, ref = graph
+    print("case learned gain", flush=True)
+    g.sched_train(False, 4, epochs=30, seed=6)
+    pred = g.sched_predict()
+    g.clique_count(5, dm.config(record_costs=1, steal=0))
+    cost = g.last_root_costs().astype(np.float64)
+    off, adj = g.csr()
+    deg = np.diff(off).astype(np.float64)
+    has = cost > 0
+    rho_model = spearman(pred[has], cost[has])
+    rho_degree = spearman(deg[has], cost[has])
+    want = O.clique_count(ref, 5)
+    t = {}
+    for name, sched in (("plain", dm.DM_SCHED_PLAIN), ("learned", dm.DM_SCHED_LEARNED)):
+        for _ in range(2):
+            assert g.clique_count(5, dm.config(sched=sched)) == want
+        t[name] = g.last_stats()["device_ms"]
+    print("k=5: spearman model %.3f vs degree %.3f; kernel plain %.3f ms, learned %.3f ms (x%.2f)"
+          % (rho_model, rho_degree, t["plain"], t["learned"], t["plain"] / t["learned"]), flush=True)
+    assert rho_model > 0.5
