@@ -14,7 +14,10 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <numeric>
+#include <tuple>
 
 #include "internal.h"
 #include "runtime.h"
@@ -164,10 +167,28 @@ int ensure_device(dm_graph *g) {
 
 // ---------------------------------------------------------------------------
 // launch shape
+static int max_lds_uncached(int vwave, int mode, int K, int ncls_single, int hist_slots);
+
 // Largest per-worker LDS stack (a multiple of 256 B, at most 16 KiB) that
 // keeps the occupancy the kernel's registers allow: the LDS of a CU (160
 // KiB) divided over that many blocks, less each block's fixed records.
 int max_lds_per_worker(int vwave, int mode, int K, int ncls_single, int hist_slots) {
+    // the answer depends only on the arguments (and the one device): memoised
+    static std::mutex mu;
+    static std::map<std::tuple<int, int, int, int, int>, int> memo;
+    const auto key = std::make_tuple(vwave, mode, K, ncls_single, hist_slots);
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        auto it = memo.find(key);
+        if (it != memo.end()) return it->second;
+    }
+    const int lds = max_lds_uncached(vwave, mode, K, ncls_single, hist_slots);
+    std::lock_guard<std::mutex> lk(mu);
+    memo[key] = lds;
+    return lds;
+}
+
+static int max_lds_uncached(int vwave, int mode, int K, int ncls_single, int hist_slots) {
     int bpc = 0;
     const size_t fixed = enum_lds_bytes(vwave, 256, K, ncls_single, hist_slots, 0);
     if (enum_occupancy(vwave, mode, 256, fixed, &bpc) != DM_OK || bpc < 1) return 32768 / (256 / vwave);
