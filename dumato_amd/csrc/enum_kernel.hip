@@ -565,24 +565,13 @@ struct Worker {
         for (int j = hi + g.gl; j < P.ne; j += W) atomicAdd(&ws.n_gt()[epm_at(P, j)], 1u);
         wsync();
         const int pnu = P.nu;
-        // Id directory of U: ids above the root split into at most 256
-        // buckets of 2^dsh ids; dir[b] = first position of bucket b.  A lookup
-        // computes its bucket and binary-searches only that bucket.
-        int32_t *dir = nullptr;
-        int dsh = 0, dnb = 0;
-        const int32_t dbase = v0 + 1;
-        if (pnu >= 32) {
-            const int32_t span = P.uid[pnu - 1] - dbase + 1;
-            while ((span >> dsh) > 256) dsh++;
-            dnb = ((span - 1) >> dsh) + 1;
-            const int64_t need = (int64_t)(dnb + 1) * 4;
-            if (lds_top + ((need + 15) & ~(int64_t)15) <= lds_lo + p.lds_per_worker) {
-                dir = (int32_t *)alloc(need);
-                for (int b = g.gl; b <= dnb; b += W)
-                    dir[b] = b == dnb ? pnu : lower_bound(P.uid, pnu, dbase + (b << dsh));
-                wsync();
-            }
-        }
+        // Fences of U, one per lane: fence(f) = U[f * pnu / W].  A lookup
+        // finds its fence interval by a search over the lanes' registers
+        // (cross-lane permutes, free of LDS bank conflicts), then searches
+        // only that interval of U.
+        const bool use_f = pnu >= 2 * W;
+        const int32_t fence = use_f ? P.uid[(int)(((int64_t)g.gl * pnu) / W)] : 0;
+        const int32_t flast = __shfl(fence, W - 1, W);  // read by every lane
         // a batch takes children until their neighbour lists hold about
         // `target` elements: few, large batches amortise the per-batch
         // class bookkeeping
@@ -616,19 +605,28 @@ struct Worker {
             for (int t0 = 0; t0 < tot; t0 += W) {
                 const int t = t0 + g.gl;
                 bool fresh = false;  // neighbour outside U: a new element
+                int32_t x = 0x7fffffff;
                 if (t < tot) {
                     while (pre[a + 1] <= t) a++;
-                    const int32_t x = p.adj[cs[a] + (t - pre[a])];
-                    int pos = pnu;
-                    if (!dir) {
-                        pos = lower_bound(P.uid, pnu, x);
-                    } else if (x >= dbase) {
-                        const int b = (x - dbase) >> dsh;
-                        if (b < dnb) {
-                            const int lo = dir[b];
-                            pos = lo + lower_bound(P.uid + lo, dir[b + 1] - lo, x);
-                        }
+                    x = p.adj[cs[a] + (t - pre[a])];
+                }
+                int flo = 0, fhi = pnu;  // interval of U holding lower_bound(x)
+                if (use_f) {
+                    // c = number of fences below x (all lanes take part)
+                    int c = 0;
+#pragma unroll
+                    for (int st = W / 2; st >= 1; st >>= 1)
+                        if (__shfl(fence, c + st - 1, W) < x) c += st;
+                    if (c == W - 1 && flast < x) c = W;
+                    if (c == 0) {
+                        fhi = 0;
+                    } else {
+                        flo = (int)(((int64_t)(c - 1) * pnu) / W) + 1;
+                        fhi = c < W ? (int)(((int64_t)c * pnu) / W) : pnu;
                     }
+                }
+                if (t < tot) {
+                    const int pos = flo + lower_bound(P.uid + flo, fhi - flo, x);
                     if (pos < pnu && P.uid[pos] == x) {
                         const uint8_t pm = upm_at(P, pos);
                         if ((pm & 0x80) && x > cc[a]) atomicAdd(&ic[a * npm + (pm & 0x7f)], 1u);
