@@ -197,6 +197,7 @@ struct Worker {
     bool abort;
     // motif leaf-phase scratch, reserved first in the LDS slice of each job:
     // cs int64[B] | ic u32[B][npm] | nn u32[B] | cc i32[B] | pre i32[B + 1]
+    // | qx i32[2W] | qa i32[2W] (lookups waiting for the search of U)
     char *lf;
     // children per leaf batch: a full worker, capped so that the per-child
     // mask counters (slots x 2^(K-2)) stay within 512 words
@@ -206,7 +207,7 @@ struct Worker {
     }
     DEV static int64_t leaf_scratch_bytes(int K) {
         const int B = leaf_slots(K);
-        return (int64_t)B * 8 + (int64_t)B * (1 << (K - 2)) * 4 + (int64_t)B * 12 + 4;
+        return (int64_t)B * 8 + (int64_t)B * (1 << (K - 2)) * 4 + (int64_t)B * 12 + 4 + 16 * W;
     }
 
     DEV Worker(const EnumParams &p_, WorkerLDS ws_, unsigned long long *sh_, char *lds_slice,
@@ -557,6 +558,8 @@ struct Worker {
         uint32_t *nn = ic + B * npm;
         int32_t *cc = (int32_t *)(nn + B);
         int32_t *pre = cc + B;
+        int32_t *qx = pre + B + 1;
+        int32_t *qa = qx + 2 * W;
         // ws.n_gt()[m]: valid elements of E above the current batch, by mask
         for (int j = g.gl; j < npm; j += W) ws.n_gt()[j] = 0;
         for (int j = g.gl; j < B * npm; j += W) ic[j] = 0;
@@ -572,6 +575,56 @@ struct Worker {
         const bool use_f = pnu >= 2 * W;
         const int32_t fence = use_f ? P.uid[(int)(((int64_t)g.gl * pnu) / W)] : 0;
         const int32_t flast = __shfl(fence, W - 1, W);  // read by every lane
+        // Membership filter of U in LDS: bit (x mod 32*words) is set for every
+        // x in U, with at least 8 bits per element where LDS allows.  Most
+        // neighbours of a child are not in U; a clear bit proves it with one
+        // LDS read, and only the rest go through the search of U.
+        uint32_t *bm = nullptr;
+        int32_t bmask = 0;
+        {
+            int words = 32;
+            while (words < 1024 && words * 4 < pnu) words <<= 1;
+            while (words >= 32 && lds_top + words * 4 > lds_lo + p.lds_per_worker) words >>= 1;
+            if (words >= 32) {
+                bm = (uint32_t *)alloc((int64_t)words * 4);
+                bmask = words * 32 - 1;
+                for (int j = g.gl; j < words; j += W) bm[j] = 0;
+                wsync();
+                for (int j = g.gl; j < pnu; j += W) {
+                    const int32_t u = P.uid[j] & bmask;
+                    atomicOr(&bm[u >> 5], 1u << (u & 31));
+                }
+                wsync();
+            }
+        }
+        // search of U for queued lookups (all lanes take part in the fence
+        // permutes; `act` lanes hold an entry): a hit adds to the child's
+        // class counter, a miss is a new element of the child
+        auto resolve = [&](int32_t x, int a, bool act) {
+            int flo = 0, fhi = pnu;  // interval of U holding lower_bound(x)
+            if (use_f) {
+                int c = 0;  // number of fences below x
+#pragma unroll
+                for (int st = W / 2; st >= 1; st >>= 1)
+                    if (__shfl(fence, c + st - 1, W) < x) c += st;
+                if (c == W - 1 && flast < x) c = W;
+                if (c == 0) {
+                    fhi = 0;
+                } else {
+                    flo = (int)(((int64_t)(c - 1) * pnu) / W) + 1;
+                    fhi = c < W ? (int)(((int64_t)c * pnu) / W) : pnu;
+                }
+            }
+            if (act) {
+                const int pos = flo + lower_bound(P.uid + flo, fhi - flo, x);
+                if (pos < pnu && P.uid[pos] == x) {
+                    const uint8_t pm = upm_at(P, pos);
+                    if ((pm & 0x80) && x > cc[a]) atomicAdd(&ic[a * npm + (pm & 0x7f)], 1u);
+                } else {
+                    atomicAdd(&nn[a], 1u);
+                }
+            }
+        };
         // a batch takes children until their neighbour lists hold about
         // `target` elements: few, large batches amortise the per-batch
         // class bookkeeping
@@ -602,38 +655,17 @@ struct Worker {
             const int tot = g.bcast(sc, take - 1);
             wsync();
             int a = 0;
+            int qn = 0;  // queued lookups (uniform)
             for (int t0 = 0; t0 < tot; t0 += W) {
                 const int t = t0 + g.gl;
-                bool fresh = false;  // neighbour outside U: a new element
                 int32_t x = 0x7fffffff;
                 if (t < tot) {
                     while (pre[a + 1] <= t) a++;
                     x = p.adj[cs[a] + (t - pre[a])];
                 }
-                int flo = 0, fhi = pnu;  // interval of U holding lower_bound(x)
-                if (use_f) {
-                    // c = number of fences below x (all lanes take part)
-                    int c = 0;
-#pragma unroll
-                    for (int st = W / 2; st >= 1; st >>= 1)
-                        if (__shfl(fence, c + st - 1, W) < x) c += st;
-                    if (c == W - 1 && flast < x) c = W;
-                    if (c == 0) {
-                        fhi = 0;
-                    } else {
-                        flo = (int)(((int64_t)(c - 1) * pnu) / W) + 1;
-                        fhi = c < W ? (int)(((int64_t)c * pnu) / W) : pnu;
-                    }
-                }
-                if (t < tot) {
-                    const int pos = flo + lower_bound(P.uid + flo, fhi - flo, x);
-                    if (pos < pnu && P.uid[pos] == x) {
-                        const uint8_t pm = upm_at(P, pos);
-                        if ((pm & 0x80) && x > cc[a]) atomicAdd(&ic[a * npm + (pm & 0x7f)], 1u);
-                    } else {
-                        fresh = true;
-                    }
-                }
+                bool need = t < tot;
+                if (bm && need) need = (bm[(x & bmask) >> 5] >> (x & 31)) & 1u;
+                const bool fresh = t < tot && !need;  // proven outside U
                 // new elements are the common case: count them per child with
                 // one ballot, the child's first lane in this round adding its
                 // segment's popcount (no same-address atomics)
@@ -646,6 +678,26 @@ struct Worker {
                         nn[a] += (uint32_t)__popcll(mf & seg);
                     }
                 }
+                // the rest wait in the queue until a full wave of them can be
+                // searched together
+                const uint64_t mq = g.ballot(need);
+                if (need) {
+                    const int o = qn + g.below(mq);
+                    qx[o] = x;
+                    qa[o] = a;
+                }
+                qn += __popcll(mq);
+                if (qn >= W) {
+                    wsync();
+                    qn -= W;
+                    resolve(qx[qn + g.gl], qa[qn + g.gl], true);
+                    wsync();
+                }
+            }
+            if (qn > 0) {
+                wsync();
+                const bool act = g.gl < qn;
+                resolve(act ? qx[g.gl] : 0x7fffffff, act ? qa[g.gl] : 0, act);
             }
             wsync();
             // per child: slot a = lane a; valid elements above it = the

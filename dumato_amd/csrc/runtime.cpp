@@ -236,7 +236,9 @@ int64_t arena_bound_for(int32_t maxdeg, int32_t max_out_, int K, bool motif) {
     if (motif) {
         for (int l = 1; l <= K - 3; l++) b += 10 * (int64_t)(l + 1) * md + 64;
         b += 4 * md + (int64_t)(K - 2) * md + 64;
-        b += 64 * 8 + 512 * 4 + 64 * 12 + 4 + 16;  // leaf-phase scratch (at most 64 slots, 512 counters)
+        // leaf-phase scratch (at most 64 slots, 512 counters, a 128-entry
+        // lookup queue)
+        b += 64 * 8 + 512 * 4 + 64 * 12 + 4 + 16 * 64 + 16;
     } else {
         int64_t mo = std::max<int64_t>(max_out_, 1);
         for (int l = 1; l <= K - 3; l++) b += 4 * mo + 16;
