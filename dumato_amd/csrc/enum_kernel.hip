@@ -254,6 +254,16 @@ struct Worker {
         fail(2);
         return nullptr;
     }
+    DEV char *alloc_glb(int64_t bytes_) {
+        int64_t b = (bytes_ + 15) & ~(int64_t)15;
+        if (g_top + b <= g_lo + p.arena_per_worker) {
+            char *r = g_top;
+            g_top += b;
+            return r;
+        }
+        fail(2);
+        return nullptr;
+    }
 
     DEV bool watchdog_expired() {
         return p.watchdog_ticks && (now_ticks() - t_start > p.watchdog_ticks);
@@ -466,10 +476,17 @@ struct Worker {
         N.mark_lds = lds_top;
         N.mark_glb = g_top;
         const int cap = pnu + q;
-        int32_t *uid = (int32_t *)alloc((int64_t)cap * 4);
-        uint8_t *upm = (uint8_t *)alloc(cap);
-        int32_t *eid = (int32_t *)alloc((int64_t)cap * 4);
-        uint8_t *epm = (uint8_t *)alloc(cap);
+        // At k >= 5 the leaf level's lists (the union of three or more
+        // neighbourhoods) go to global memory when they would take more than
+        // half the free LDS: the leaf phase reads them rarely and needs the
+        // LDS for its membership filter.  (At k = 4 they stay in LDS: short
+        // lists, and moving them measured slower.)
+        const bool lg = l1 == K - 3 && l1 >= 2 &&
+                        (int64_t)cap * 10 > (lds_lo + p.lds_per_worker - lds_top) / 2;
+        int32_t *uid = (int32_t *)(lg ? alloc_glb((int64_t)cap * 4) : alloc((int64_t)cap * 4));
+        uint8_t *upm = (uint8_t *)(lg ? alloc_glb(cap) : alloc(cap));
+        int32_t *eid = (int32_t *)(lg ? alloc_glb((int64_t)cap * 4) : alloc((int64_t)cap * 4));
+        uint8_t *epm = (uint8_t *)(lg ? alloc_glb(cap) : alloc(cap));
         char *tmark_l = lds_top, *tmark_g = g_top;
         int32_t *nw = (int32_t *)alloc((int64_t)(q > 0 ? q : 1) * 4);
         uint8_t *hit = (uint8_t *)alloc(pnu > 0 ? pnu : 1);
