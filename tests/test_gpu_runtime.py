@@ -70,6 +70,30 @@ def test_small_lds_stacks_spill_to_the_arena(rmat, golden, lds):
         assert rmat.clique_count(k, cfg(lds_bytes=lds, vwave=64, steal_min=2)) == want[str(k)]
 
 
+def hist_of(h):
+    return {str(i): int(c) for i, c in enumerate(h) if c}
+
+
+@pytest.mark.parametrize("lds", [256, 2048])
+def test_small_lds_motif_leaf_paths(golden, lds):
+    # little or no LDS for the motif leaf phase: its scratch and the level
+    # lists go to the arena, the membership filter of U shrinks or is left
+    # out (every lookup then goes through the search queue); histograms must
+    # not change
+    cl = W.chunglu_18k()
+    want = golden["chunglu_18k"]["motif"]["4"]
+    for w in (16, 64):
+        print("case lds=%d motif k=4 chunglu vwave=%d" % (lds, w), flush=True)
+        tot, h = cl.motif_count(4, cfg(lds_bytes=lds, vwave=w, steal_min=2))
+        assert tot == want["total"] and hist_of(h) == want["hist"]
+    ba = W.ba_100k_planted()
+    for rng, want in golden["ba_100k_planted"]["motif5_root_ranges"].items():
+        a, b = map(int, rng.split("-"))
+        print("case lds=%d motif k=5 ba roots %s" % (lds, rng), flush=True)
+        tot, h = ba.motif_count(5, cfg(lds_bytes=lds, vwave=64, root_begin=a, root_end=b))
+        assert tot == want["total"] and hist_of(h) == want["hist"]
+
+
 def test_heavy_donation_traffic_keeps_counts(rmat, golden):
     # donate as often and as finely as the runtime allows
     want = golden["rmat_s17"]["clique"]
