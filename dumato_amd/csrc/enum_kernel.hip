@@ -24,9 +24,10 @@
 //     reference's extensions after canonicalFilter2; the mask bits are the
 //     reference's induction bits (forward_induce / aggregate_pattern).
 //   * The last level is never materialised: for each child c of level K-3,
-//     one pass over N(c) (binary search of each element in U_{K-3}) gives, per
-//     mask value, how many leaves are adjacent to c, and the class of every
-//     leaf follows from (mask, adjacency to c) through the class table.
+//     one pass over N(c) (an LDS membership bitmap of U_{K-3} first; the
+//     elements it does not rule out are searched in U_{K-3}) gives, per mask
+//     value, how many leaves are adjacent to c, and the class of every leaf
+//     follows from (mask, adjacency to c) through the class table.
 //   * Work: static per-worker queues of (root, lo, hi) jobs, stealing of
 //     other workers' static jobs, and donation of the lowest level's
 //     remaining children to a device queue when workers are idle (the
