@@ -643,10 +643,11 @@ struct Worker {
                 }
             }
         };
-        // a batch takes children until their neighbour lists hold about
-        // `target` elements: few, large batches amortise the per-batch
-        // class bookkeeping
-        const int target = 32 * W;
+        // a batch takes children (at most B) until their neighbour lists hold
+        // about `target` elements: few, large batches amortise the per-batch
+        // class bookkeeping; the bound keeps the leaf donation checks, one
+        // per batch, frequent when the children are hubs
+        const int target = 256 * W;
         int i = hi - 1;
         while (i >= lo) {
             if (check_abort()) return;
