@@ -10,8 +10,8 @@ Run in a container where the reference tree is mounted (default
 * datasets/test.edgelist and datasets/citeseer.edgelist, copied verbatim to
   tests/golden/datasets/.
 
-The committed fixtures are what the tests use, so the GPU box (which has no
-reference tree) runs from them.
+The committed fixtures are what the tests use, so a machine without the
+reference tree runs from them.
 """
 import hashlib
 import json
