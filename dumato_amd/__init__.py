@@ -100,6 +100,11 @@ SYMBOLS = [
     ("dm_batch_free", None, [GP]),
     ("dm_mlp_grad", ctypes.c_int, [f32p, f32p, f32p, ctypes.c_int, f32p, f32p, f32p]),
     ("dm_mlp_adam", ctypes.c_int, [f32p, f32p, f32p, ctypes.c_int, ctypes.c_int, ctypes.c_float, f32p]),
+    ("dm_validate_counter", ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, ctypes.c_uint64,
+                                           ctypes.POINTER(ctypes.c_int)]),
+    ("dm_validate_pattern", ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, u64p, ctypes.c_size_t,
+                                           ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int), u64p,
+                                           ctypes.POINTER(ctypes.c_size_t)]),
 ]
 
 
@@ -266,6 +271,26 @@ def pattern_table(k):
     codes = np.zeros(ncls, np.uint64)
     _check(lib().dm_pattern_table(int(k), _ptr(q2c, u16p), nq, _ptr(codes, u64p), ncls))
     return q2c, codes
+
+
+def validate_counter(graph_path, k, result):
+    """True when "<graph_path>.<k>.aggregateCounter" (Peregrine output) sums to
+    result; DuMatoCPU::validateAggregateCounter, src/main/DuMatoCPU.cu:564-595.
+    Raises DMError when the file cannot be opened."""
+    m = ctypes.c_int()
+    _check(lib().dm_validate_counter(os.fsencode(graph_path), int(k), int(result), ctypes.byref(m)))
+    return bool(m.value)
+
+
+def validate_pattern(graph_path, k, hist):
+    """(count_match, pattern_match) of hist against "<graph_path>.<k>.aggregatePattern":
+    equal totals, and equal sorted multisets of the positive counts (class ids are
+    not compared); DuMatoCPU::validateAggregatePattern, src/main/DuMatoCPU.cu:597-661."""
+    h = np.ascontiguousarray(hist, dtype=np.uint64)
+    cm, pm = ctypes.c_int(), ctypes.c_int()
+    _check(lib().dm_validate_pattern(os.fsencode(graph_path), int(k), _ptr(h, u64p), len(h),
+                                     ctypes.byref(cm), ctypes.byref(pm), None, None))
+    return bool(cm.value), bool(pm.value)
 
 
 MLP_PARAMS = 4801

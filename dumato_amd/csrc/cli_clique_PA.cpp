@@ -52,6 +52,15 @@ int main(int argc, const char **argv) {
     dm_stats st;
     dm_last_stats(g, &st);
     printf("Result: %lu.\n", (unsigned long)count);
+    // DuMatoCPU::validateAggregateCounter (src/main/DuMatoCPU.cu:564-595),
+    // which clique_PA never calls; DM_VALIDATE=1 calls it here.
+    if (const char *v = getenv("DM_VALIDATE"); v && atoi(v)) {
+        int match = 0;
+        if (dm_validate_counter(path, k, count, &match) != DM_OK)
+            fprintf(stderr, "Error opening the validation file of aggregate counter.\n");
+        else
+            printf("validateAggregateCounter: %s\n", match ? "MATCH" : "MISMATCH");
+    }
     double gpu = st.device_ms / 1e3;
     printf("Time (GPU): %f secs.\n", gpu);
     printf("Time (LB): %f secs,%f%%\n", 0.0, 0.0);

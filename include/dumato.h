@@ -194,6 +194,23 @@ int dm_batch_autotune(dm_batch *b, const dm_query *queries, size_t nq,
 int dm_batch_last_stats(const dm_batch *b, dm_stats *st);
 void dm_batch_free(dm_batch *b);
 
+/* ---- validation against Peregrine output files ---- */
+/* Reads "<graph_path>.<k>.aggregateCounter" and sets *match = 1 when the sum
+ * of its '['-line values equals result.  Replaces
+ * DuMatoCPU::validateAggregateCounter (src/main/DuMatoCPU.cu:564-595).
+ * DM_ERR_IO when the file cannot be opened (the reference prints an error and
+ * returns).  Values are 64-bit (the reference's atoi wraps above 2^31-1). */
+int dm_validate_counter(const char *graph_path, int k, uint64_t result, int *match);
+/* Reads "<graph_path>.<k>.aggregatePattern"; *count_match = 1 when its total
+ * equals sum(hist), *pattern_match = 1 when the sorted multiset of its positive
+ * values equals that of hist's positive slots (class ids are not compared).
+ * file_total / file_patterns (nullable) receive the file's total and number of
+ * positive patterns.  Replaces DuMatoCPU::validateAggregatePattern
+ * (src/main/DuMatoCPU.cu:597-661). */
+int dm_validate_pattern(const char *graph_path, int k, const uint64_t *hist, size_t hist_len,
+                        int *count_match, int *pattern_match, uint64_t *file_total,
+                        size_t *file_patterns);
+
 const char *dm_last_error(void);
 int dm_version(void);
 
