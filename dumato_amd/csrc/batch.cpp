@@ -31,15 +31,15 @@ namespace dm {
 
 struct BatchPlan {
     std::vector<dm_query> queries;
-    int vwave = 0, lds = 0, steal = 1, steal_min = 2, steal_period = 2, bitmap = 1, blocks = 0;
+    int vwave = 0, lds = 0, steal = 1, steal_min = 2, steal_period = 2, bitmap = 1, blocks = 0, threads = 256;
     double watchdog_s = 0;
-    bool same(const std::vector<dm_query> &q, const dm_config &c, int vw, int l, int sm) const {
+    bool same(const std::vector<dm_query> &q, const dm_config &c, int vw, int l, int sm, int thr) const {
         if (q.size() != queries.size()) return false;
         for (size_t i = 0; i < q.size(); i++)
             if (q[i].graph != queries[i].graph || q[i].motif != queries[i].motif || q[i].k != queries[i].k)
                 return false;
         return vw == vwave && l == lds && c.steal == steal && sm == steal_min && c.steal_period == steal_period &&
-               c.clique_bitmap == bitmap && c.blocks == blocks && c.watchdog_s == watchdog_s;
+               c.clique_bitmap == bitmap && c.blocks == blocks && c.watchdog_s == watchdog_s && thr == threads;
     }
 };
 
@@ -50,10 +50,22 @@ struct dm_batch {
     std::vector<int32_t> vbase;      // first union id of each graph
     std::vector<int32_t> gn;         // vertices per graph
     dm::HostCSR u;                   // union CSR (degree-rank ids per graph)
-    std::vector<int32_t> ostart;
+    std::vector<int64_t> ostart;
     int32_t max_out = 0;
-    // device
+    // device: the union CSR (and the device context whose stream, queue,
+    // epoch and class tables the batch uses), and the workspaces the captured
+    // hipGraph points to, which are the batch's own
     dm::DeviceGraph *dev = nullptr;
+    char *d_arena = nullptr;
+    size_t arena_cap = 0;
+    int64_t *d_sq_off = nullptr;
+    size_t sq_off_cap = 0;
+    int32_t *d_sq_jobs = nullptr;
+    size_t sq_jobs_cap = 0;
+    int32_t *d_sq_cur = nullptr;
+    size_t sq_cur_cap = 0;
+    unsigned long long *d_results = nullptr;
+    size_t results_cap = 0;
     dm::QueryDesc *d_qd = nullptr;
     size_t qd_cap = 0;
     unsigned long long *d_done = nullptr;
@@ -89,16 +101,19 @@ static void drop_graph(dm_batch *b) {
 }
 
 static int batch_device(dm_batch *b) {
-    if (b->dev) return DM_OK;
-    // reuse the single-graph device setup on a stand-in handle of the union
-    dm_graph tmp;
-    tmp.rcsr = b->u;
-    tmp.ostart = b->ostart;
-    tmp.max_out = b->max_out;
-    int r = ensure_device(&tmp);
-    b->dev = tmp.dev;
-    tmp.dev = nullptr;
-    return r;
+    if (b->dev)
+        return hipSetDevice(b->dev->ctx->device) == hipSuccess ? DM_OK : set_error(DM_ERR_HIP, "hipSetDevice");
+    DeviceCtx *c = nullptr;
+    int r = device_ctx(&c);
+    if (r) return r;
+    DeviceGraph *d = new DeviceGraph();
+    d->ctx = c;
+    if ((r = upload_csr(b->u, b->ostart, d))) {
+        free_device_graph(d);
+        return r;
+    }
+    b->dev = d;
+    return DM_OK;
 }
 
 // Range jobs: each query's roots in chunks; a chunk's cost estimate is the
@@ -146,7 +161,8 @@ static void batch_partition(const dm_batch *b, const std::vector<dm_query> &qs, 
 // queries first), so that each family gets its specialised kernel and
 // occupancy; the two share the queue, the result slots and the clock.
 static int batch_build(dm_batch *b, const std::vector<dm_query> &qs, const dm_config &cfg) {
-    DeviceGraph *d = b->dev;
+    DeviceGraph *dg = b->dev;
+    DeviceCtx *d = dg->ctx;
     int kmax_f[2] = {2, 2}, hslots = 0;
     bool has[2] = {false, false};
     for (const dm_query &Q : qs) {
@@ -166,10 +182,13 @@ static int batch_build(dm_batch *b, const std::vector<dm_query> &qs, const dm_co
     // launch shape: width / LDS from the config (autotuned values cached in the plan)
     int vw = cfg.vwave ? cfg.vwave : (b->plan.vwave ? b->plan.vwave : 64);
     if (vw != 8 && vw != 16 && vw != 32 && vw != 64) return set_error(DM_ERR_ARG, "vwave must be 8/16/32/64");
+    int threads = 256;
+    int r = block_threads(cfg, &threads);
+    if (r) return r;
     // 0: each phase takes the largest LDS stack its occupancy allows
     const int lds = cfg.lds_bytes > 0 ? (cfg.lds_bytes + 15) & ~15 : 0;
     const int sm = cfg.steal_min ? cfg.steal_min : (b->tuned_steal_min ? b->tuned_steal_min : 2);
-    if (b->exec && b->plan.same(qs, cfg, vw, lds, sm)) return DM_OK;
+    if (b->exec && b->plan.same(qs, cfg, vw, lds, sm, threads)) return DM_OK;
     drop_graph(b);
     // phases: (mode, kmax, per-worker class slots) per family present; a
     // single query keeps the batch's per-worker counters (mode 2)
@@ -195,7 +214,6 @@ static int batch_build(dm_batch *b, const std::vector<dm_query> &qs, const dm_co
             }
         ph.push_back(std::move(P));
     }
-    int r = DM_OK;
     int max_workers = 0;
     size_t sq_total = 0, jobs_total = 0;
     for (Phase &P : ph) {
@@ -203,13 +221,13 @@ static int batch_build(dm_batch *b, const std::vector<dm_query> &qs, const dm_co
         // the query's classes) instead of per-worker ones
         const int ncls_single = qs.size() == 1 ? P.wslots : 0;
         const int hs = qs.size() == 1 ? 0 : P.wslots;
-        P.lds = lds > 0 ? lds : max_lds_per_worker(vw, P.mode, P.kmax, ncls_single, hs);
-        P.lds_total = enum_lds_bytes(vw, 256, P.kmax, ncls_single, hs, P.lds);
+        P.lds = lds > 0 ? lds : max_lds_per_worker(vw, threads, P.mode, P.kmax, ncls_single, hs);
+        P.lds_total = enum_lds_bytes(vw, threads, P.kmax, ncls_single, hs, P.lds);
         int bpc = 0;
-        if ((r = enum_occupancy(vw, P.mode, 256, P.lds_total, &bpc))) return r;
+        if ((r = enum_occupancy(vw, P.mode, threads, P.lds_total, &bpc))) return r;
         if (bpc < 1) return set_error(DM_ERR_ARG, "batch launch does not fit one CU (LDS %zu B)", P.lds_total);
         P.blocks = cfg.blocks > 0 ? cfg.blocks : bpc * d->num_cu;
-        P.workers = P.blocks * (256 / vw);
+        P.workers = P.blocks * (threads / vw);
         batch_partition(b, P.qs, P.workers, P.sq_off, P.sq_jobs);
         // job records name queries by their index in the whole batch
         for (size_t j = 0; j < P.sq_jobs.size(); j += 4) P.sq_jobs[j + 3] = P.qidx[P.sq_jobs[j + 3]];
@@ -225,12 +243,11 @@ static int batch_build(dm_batch *b, const std::vector<dm_query> &qs, const dm_co
     int64_t arena = 256;
     for (const dm_query &Q : qs)
         arena = std::max(arena, arena_bound_for(b->u.maxdeg, b->max_out, Q.k, Q.motif != 0));
-    if ((r = dev_grow(d->d_arena, d->arena_cap, (size_t)arena * max_workers))) return r;
-    if ((r = dev_grow(d->d_sq_off, d->sq_off_cap, sq_total))) return r;
-    if ((r = dev_grow(d->d_sq_jobs, d->sq_jobs_cap, std::max<size_t>(jobs_total, 4)))) return r;
-    if ((r = dev_grow(d->d_sq_cur, d->sq_cur_cap, (size_t)max_workers))) return r;
-    if ((r = ensure_queue(d))) return r;
-    if ((r = dev_grow(d->d_results, d->results_cap, b->nres))) return r;
+    if ((r = dev_grow(b->d_arena, b->arena_cap, (size_t)arena * max_workers))) return r;
+    if ((r = dev_grow(b->d_sq_off, b->sq_off_cap, sq_total))) return r;
+    if ((r = dev_grow(b->d_sq_jobs, b->sq_jobs_cap, std::max<size_t>(jobs_total, 4)))) return r;
+    if ((r = dev_grow(b->d_sq_cur, b->sq_cur_cap, (size_t)max_workers))) return r;
+    if ((r = dev_grow(b->d_results, b->results_cap, b->nres))) return r;
     if ((r = dev_grow(b->d_qd, b->qd_cap, qs.size()))) return r;
     if ((r = dev_grow(b->d_done, b->done_cap, qs.size()))) return r;
     if ((r = dev_grow(b->d_start, b->start_cap, 1))) return r;
@@ -246,16 +263,10 @@ static int batch_build(dm_batch *b, const std::vector<dm_query> &qs, const dm_co
         D.graph = Q.graph;
         b->q_graph[i] = Q.graph;
         if (Q.motif) {
-            const PatternTable *t = pattern_table(Q.k);
-            D.ncls = t->ncls;
-            if (!d->d_lut[Q.k]) {
-                size_t c = 0;
-                if ((r = dev_grow(d->d_lut[Q.k], c, t->q2c.size()))) return r;
-                HIPCHK(hipMemcpy(d->d_lut[Q.k], t->q2c.data(), t->q2c.size() * 2, hipMemcpyHostToDevice));
-            }
-            D.lut = d->d_lut[Q.k];
+            D.ncls = pattern_table(Q.k)->ncls;
+            if ((r = device_lut(d, Q.k, &D.lut))) return r;
         }
-        D.out = d->d_results + b->res_off[i];
+        D.out = b->d_results + b->res_off[i];
         D.done_ticks = b->d_done + i;
     }
     HIPCHK(hipMemcpy(b->d_qd, qd.data(), qd.size() * sizeof(QueryDesc), hipMemcpyHostToDevice));
@@ -263,29 +274,29 @@ static int batch_build(dm_batch *b, const std::vector<dm_query> &qs, const dm_co
     std::vector<EnumParams> params(ph.size());
     for (size_t pi = 0; pi < ph.size(); pi++) {
         Phase &P = ph[pi];
-        HIPCHK(hipMemcpy(d->d_sq_off + o_off, P.sq_off.data(), P.sq_off.size() * 8, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(b->d_sq_off + o_off, P.sq_off.data(), P.sq_off.size() * 8, hipMemcpyHostToDevice));
         if (!P.sq_jobs.empty())
-            HIPCHK(hipMemcpy(d->d_sq_jobs + o_jobs, P.sq_jobs.data(), P.sq_jobs.size() * 4, hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(b->d_sq_jobs + o_jobs, P.sq_jobs.data(), P.sq_jobs.size() * 4, hipMemcpyHostToDevice));
         EnumParams &p = params[pi];
         memset(&p, 0, sizeof(p));
-        p.off = d->d_off;
-        p.adj = d->d_adj;
-        p.ostart = d->d_ostart;
+        p.off = dg->d_off;
+        p.adj = dg->d_adj;
+        p.ostart = dg->d_ostart;
         p.n = b->u.n;
         p.nq = (int32_t)qs.size();
         p.qd = b->d_qd;
         p.kmax = P.kmax;
         p.hist_slots = P.wslots;
         p.nworkers = P.workers;
-        p.sq_off = d->d_sq_off + o_off;
-        p.sq_jobs = d->d_sq_jobs + o_jobs;
-        p.sq_cur = d->d_sq_cur;
+        p.sq_off = b->d_sq_off + o_off;
+        p.sq_jobs = b->d_sq_jobs + o_jobs;
+        p.sq_cur = b->d_sq_cur;
         p.q = d->d_q;
         p.q_cap = (uint32_t)d->q_cap_alloc;
         p.epoch = d->d_epoch;
         p.start_ticks = b->d_start;
         p.sc = b->d_sc + pi;
-        p.arena = d->d_arena;
+        p.arena = b->d_arena;
         p.arena_per_worker = arena;
         p.lds_per_worker = P.lds;
         p.steal = cfg.steal ? 1 : 0;
@@ -308,9 +319,9 @@ static int batch_build(dm_batch *b, const std::vector<dm_query> &qs, const dm_co
         for (size_t w = 0; w + 1 < P.sq_off.size(); w++) queues += P.sq_off[w + 1] > P.sq_off[w] ? 1 : 0;
         EnumParams ip = params[pi];
         if (pi > 0) ip.start_ticks = nullptr;
-        rr = enum_init_launch(ip, (unsigned long long)njobs, queues, d->d_epoch, pi == 0 ? d->d_results : nullptr,
+        rr = enum_init_launch(ip, (unsigned long long)njobs, queues, d->d_epoch, pi == 0 ? b->d_results : nullptr,
                               pi == 0 ? b->nres : 0, pi == 0 ? b->d_done : nullptr, pi == 0 ? (int)qs.size() : 0, s);
-        if (!rr && njobs > 0) rr = enum_launch(params[pi], P.mode, vw, P.blocks, 256, P.lds_total, s);
+        if (!rr && njobs > 0) rr = enum_launch(params[pi], P.mode, vw, P.blocks, threads, P.lds_total, s);
     }
     hipError_t ce = hipStreamEndCapture(s, &b->graph);
     if (rr) return rr;
@@ -326,6 +337,7 @@ static int batch_build(dm_batch *b, const std::vector<dm_query> &qs, const dm_co
     b->plan.bitmap = cfg.clique_bitmap;
     b->plan.blocks = cfg.blocks;
     b->plan.watchdog_s = cfg.watchdog_s;
+    b->plan.threads = threads;
     b->last.vwave = vw;
     b->last.workers = max_workers;
     b->last.blocks = ph.back().blocks;
@@ -336,11 +348,13 @@ static int batch_build(dm_batch *b, const std::vector<dm_query> &qs, const dm_co
 static int batch_run(dm_batch *b, const std::vector<dm_query> &qs, const dm_config &cfg, uint64_t *results,
                      size_t results_len, uint64_t *offsets, double *latency_ms) {
     const double t0 = now_ms();
-    int r = batch_device(b);
+    int r = check_config(cfg);
     if (r) return r;
+    if ((r = batch_device(b))) return r;
+    DeviceCtx *d = b->dev->ctx;
+    std::lock_guard<std::recursive_mutex> lock(d->mu);
     if ((r = batch_build(b, qs, cfg))) return r;
     const double t1 = now_ms();
-    DeviceGraph *d = b->dev;
     hipStream_t s = d->stream;
     if (results && results_len < b->nres) return set_error(DM_ERR_ARG, "results need %zu slots", b->nres);
     HIPCHK(hipEventRecord(d->ev0, s));
@@ -350,7 +364,7 @@ static int batch_run(dm_batch *b, const std::vector<dm_query> &qs, const dm_conf
     HIPCHK(hipMemcpyAsync(scs, b->d_sc, sizeof(SchedCounters) * b->nphases, hipMemcpyDeviceToHost, s));
     std::vector<unsigned long long> res(b->nres), done(qs.size());
     unsigned long long start = 0;
-    HIPCHK(hipMemcpyAsync(res.data(), d->d_results, b->nres * 8, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(res.data(), b->d_results, b->nres * 8, hipMemcpyDeviceToHost, s));
     HIPCHK(hipMemcpyAsync(done.data(), b->d_done, qs.size() * 8, hipMemcpyDeviceToHost, s));
     HIPCHK(hipMemcpyAsync(&start, b->d_start, 8, hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
@@ -373,10 +387,7 @@ static int batch_run(dm_batch *b, const std::vector<dm_query> &qs, const dm_conf
         sc1.t_last_job = c.t_last_job;
         sc1.t_exit = c.t_exit;
     }
-    if (sc1.error) {
-        const char *why = sc1.error == 1 ? "watchdog expired" : sc1.error == 2 ? "stack arena overflow" : "invalid job";
-        return set_error(DM_ERR_DEVICE, "batch kernel aborted: %s", why);
-    }
+    if (sc1.error) return device_error(sc1.error, "batch kernel");
     uint64_t total = 0;
     for (size_t i = 0; i < b->nres; i++) total += res[i];
     if (results)
@@ -449,7 +460,7 @@ int dm_batch_create(dm_graph *const *graphs, size_t ngraphs, dm_batch **out) {
         b->gn.push_back(g->rcsr.n);
         for (int32_t v = 0; v < g->rcsr.n; v++) {
             b->u.off[vb + v + 1] = eb + g->rcsr.off[v + 1];
-            b->ostart[vb + v] = (int32_t)(eb + g->ostart[v]);
+            b->ostart[vb + v] = eb + g->ostart[v];
         }
         for (int64_t e = 0; e < g->rcsr.nnz(); e++) b->u.adj[eb + e] = vb + g->rcsr.adj[e];
         b->u.maxdeg = std::max(b->u.maxdeg, g->rcsr.maxdeg);
@@ -568,20 +579,11 @@ int dm_batch_num_results(const dm_batch *b, const dm_query *queries, size_t nq, 
 void dm_batch_free(dm_batch *b) {
     if (!b) return;
     drop_graph(b);
-    if (b->d_qd) (void)hipFree(b->d_qd);
-    if (b->d_done) (void)hipFree(b->d_done);
-    if (b->d_start) (void)hipFree(b->d_start);
-    if (b->d_sc) (void)hipFree(b->d_sc);
-    if (b->dev) {
-        dm_graph tmp;
-        tmp.dev = b->dev;
-        b->dev = nullptr;
-        // graph handle destructor path frees the device buffers
-        dm_graph *h = new dm_graph();
-        h->dev = tmp.dev;
-        tmp.dev = nullptr;
-        dm_graph_free(h);
-    }
+    void *bufs[] = {b->d_arena, b->d_sq_off, b->d_sq_jobs, b->d_sq_cur, b->d_results,
+                    b->d_qd, b->d_done, b->d_start, b->d_sc};
+    for (void *p : bufs)
+        if (p) (void)hipFree(p);
+    free_device_graph(b->dev);
     delete b;
 }
 

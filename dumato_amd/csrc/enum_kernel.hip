@@ -43,6 +43,7 @@ namespace dm {
 
 #define DEV __device__ __forceinline__
 
+// Largest block (4 waves); a launch may use 1, 2 or 4 waves per block.
 constexpr int kThreads = 256;
 // Waves per SIMD each instantiation is compiled for: 4 (at most 128 VGPRs)
 // for the full-wave motif kernel, which fits that without spilling, 3 (168
@@ -1181,7 +1182,12 @@ struct JobHead {
 
 template <int W>
 DEV bool take_static(const EnumParams &p, const Grp<W> &g, int w, const WorkerLDS &ws, JobHead &jh) {
-    int64_t b = p.sq_off[w], e = p.sq_off[w + 1];
+    // strided partition (no job arrays): worker w holds roots n-1-w,
+    // n-1-w-nworkers, ...
+    const bool strided = p.sq_jobs == nullptr;
+    int64_t b = 0, e = 0;
+    if (strided) e = w < p.n ? (int64_t)((p.n - 1 - w) / p.nworkers) + 1 : 0;
+    else b = p.sq_off[w], e = p.sq_off[w + 1];
     int got = 0, r = 0, lo = 0, hi = 0, q = 0;
     if (g.gl == 0 && b < e) {
         int cur = __hip_atomic_load(&p.sq_cur[w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -1189,11 +1195,18 @@ DEV bool take_static(const EnumParams &p, const Grp<W> &g, int w, const WorkerLD
             int j = atomicAdd(&p.sq_cur[w], 1);
             if (b + j < e) {
                 got = 1;
-                const int32_t *t = p.sq_jobs + 4 * (b + j);
-                r = t[0];
-                lo = t[1];
-                hi = t[2];
-                q = t[3];
+                if (strided) {
+                    r = p.n - 1 - (w + j * p.nworkers);
+                    lo = -1;
+                    hi = r + 1;
+                    q = 0;
+                } else {
+                    const int32_t *t = p.sq_jobs + 4 * (b + j);
+                    r = t[0];
+                    lo = t[1];
+                    hi = t[2];
+                    q = t[3];
+                }
                 // whoever claims a queue's last job retires the queue
                 if (b + j == e - 1) atomicSub(&p.sc->static_left, 1);
             }
@@ -1316,7 +1329,8 @@ template <int W, int MODE>
 __global__ __launch_bounds__(kThreads, (min_waves<W, MODE>())) void enum_kernel(EnumParams p) {
     extern __shared__ __align__(16) char smem[];
     Grp<W> g;
-    const int wpb = kThreads / W;
+    const int nthr = (int)blockDim.x;  // 64, 128 or 256
+    const int wpb = nthr / W;
     const int wib = uni<W>(threadIdx.x / W);
     const int wid = blockIdx.x * wpb + wib;
     // LDS: single-query class counters of the block | per-worker records |
@@ -1332,8 +1346,8 @@ __global__ __launch_bounds__(kThreads, (min_waves<W, MODE>())) void enum_kernel(
     unsigned long long *whist = (unsigned long long *)base + (size_t)wib * wslots;
     base += (size_t)wpb * wslots * 8;
     char *slice = base + (int64_t)wib * p.lds_per_worker;
-    for (int i = threadIdx.x; i < bslots; i += kThreads) bhist[i] = 0;
-    for (int i = threadIdx.x; i < wpb * wslots; i += kThreads) ((unsigned long long *)(smem + (size_t)bslots * 8 + wpb * wsb))[i] = 0;
+    for (int i = threadIdx.x; i < bslots; i += nthr) bhist[i] = 0;
+    for (int i = threadIdx.x; i < wpb * wslots; i += nthr) ((unsigned long long *)(smem + (size_t)bslots * 8 + wpb * wsb))[i] = 0;
     if (threadIdx.x == 0) {
         const unsigned long long t = wall_clock64();
         atomicMin(&p.sc->t_start, t);
@@ -1451,7 +1465,7 @@ __global__ __launch_bounds__(kThreads, (min_waves<W, MODE>())) void enum_kernel(
     }
     __syncthreads();
     if (single && bslots) {
-        for (int i = threadIdx.x; i < p.qd[0].ncls; i += kThreads)
+        for (int i = threadIdx.x; i < p.qd[0].ncls; i += nthr)
             if (bhist[i]) atomicAdd(&p.qd[0].out[i], bhist[i]);
     }
     // Per-worker statistics (held by lane 0 of each worker) reduced over the
@@ -1480,7 +1494,7 @@ __global__ __launch_bounds__(kThreads, (min_waves<W, MODE>())) void enum_kernel(
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        for (int w = 1; w < kThreads / 64; w++) {
+        for (int w = 1; w < nthr / 64; w++) {
             for (int i = 0; i < NF; i++) f[i] = i < 5 ? f[i] + red[w][i] : (red[w][i] > f[i] ? red[w][i] : f[i]);
         }
         if (f[0]) atomicAdd(&p.sc->alg_bytes, f[0]);
@@ -1567,9 +1581,11 @@ static const void *kernel_for(int vwave, int mode) {
     }
 }
 
+static bool threads_ok(int threads) { return threads == 64 || threads == 128 || threads == 256; }
+
 int enum_occupancy(int vwave, int mode, int threads, size_t lds, int *blocks_per_cu) {
     const void *k = kernel_for(vwave, mode);
-    if (!k || threads != kThreads) return set_error(DM_ERR_ARG, "unsupported launch shape");
+    if (!k || !threads_ok(threads) || threads < vwave) return set_error(DM_ERR_ARG, "unsupported launch shape");
     if (lds > 160 * 1024) return set_error(DM_ERR_ARG, "LDS request %zu exceeds 160 KiB", lds);
     if (lds > 64 * 1024) {
         hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
@@ -1584,7 +1600,7 @@ int enum_occupancy(int vwave, int mode, int threads, size_t lds, int *blocks_per
 
 int enum_launch(const EnumParams &p, int mode, int vwave, int blocks, int threads, size_t lds, void *stream) {
     const void *k = kernel_for(vwave, mode);
-    if (!k || threads != kThreads) return set_error(DM_ERR_ARG, "unsupported launch shape");
+    if (!k || !threads_ok(threads) || threads < vwave) return set_error(DM_ERR_ARG, "unsupported launch shape");
     EnumParams copy = p;
     void *args[] = {&copy};
     hipError_t e = hipLaunchKernel(k, dim3(blocks), dim3(threads), args, lds, (hipStream_t)stream);

@@ -123,7 +123,7 @@ struct EnumParams {
     // N(v) above v
     const int64_t *off;
     const int32_t *adj;
-    const int32_t *ostart;
+    const int64_t *ostart;
     int32_t n;
     int32_t nq;                       // queries
     const QueryDesc *qd;
@@ -134,6 +134,10 @@ struct EnumParams {
     // static partition: worker w owns records sq_jobs[4*sq_off[w] .. 4*sq_off[w+1]):
     //   (root, lo, hi, q)   children [lo, hi) of root (q = query), or
     //   (r0, -1, r1, q)     every root of [r0, r1), all children
+    // sq_jobs == nullptr: the strided partition of one query, with no arrays:
+    // worker w's j-th job is root r = n-1 - (w + j*nworkers) as the range
+    // (r, -1, r+1, 0) -- the reference's v -> warp v % W deal, heaviest
+    // (highest degree rank) first
     const int64_t *sq_off;
     const int32_t *sq_jobs;
     int32_t *sq_cur;

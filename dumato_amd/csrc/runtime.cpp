@@ -15,6 +15,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <numeric>
 #include <tuple>
@@ -78,15 +79,78 @@ int graph_finish(dm_graph *g) {
         for (int64_t i = 0; i < d; i++) rc.adj[b + i] = g->perm[c.adj[c.off[v] + i]];
         std::sort(rc.adj.begin() + b, rc.adj.begin() + b + d);
         rc.off[r + 1] = b + d;
-        int64_t up = std::upper_bound(rc.adj.begin() + b, rc.adj.begin() + b + d, r) - rc.adj.begin();
-        g->ostart[r] = (int32_t)up;
+        const int64_t up = std::upper_bound(rc.adj.begin() + b, rc.adj.begin() + b + d, r) - rc.adj.begin();
+        g->ostart[r] = up;
         max_out = std::max<int32_t>(max_out, (int32_t)(b + d - up));
     }
     g->max_out = max_out;
     return DM_OK;
 }
 
-static void device_free(DeviceGraph *d) {
+// ---------------------------------------------------------------------------
+// device contexts (one per GPU, kept until the process ends)
+static std::mutex g_ctx_mu;
+static std::map<int, std::unique_ptr<DeviceCtx>> g_ctx;
+
+int device_ctx(DeviceCtx **out) {
+    int ndev = 0;
+    hipError_t e = hipGetDeviceCount(&ndev);
+    if (e != hipSuccess || ndev < 1) return set_error(DM_ERR_HIP, "no HIP device available");
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(g_ctx_mu);
+    auto it = g_ctx.find(dev);
+    if (it != g_ctx.end()) {
+        *out = it->second.get();
+        return DM_OK;
+    }
+    std::unique_ptr<DeviceCtx> c(new DeviceCtx());
+    c->device = dev;
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, dev));
+    c->num_cu = prop.multiProcessorCount;
+    int rate = 0;
+    if (hipDeviceGetAttribute(&rate, hipDeviceAttributeWallClockRate, dev) != hipSuccess || rate <= 0) rate = 100000;
+    c->wall_khz = rate;
+    snprintf(c->arch, sizeof(c->arch), "%s", prop.gcnArchName);
+    HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    HIPCHK(hipEventCreate(&c->ev0));
+    HIPCHK(hipEventCreate(&c->ev1));
+    // donated-job queue: 2^20 slots; epoch 0 never matches a launch
+    const size_t qcap = (size_t)1 << 20;
+    int r = dev_grow(c->d_q, c->q_cap_alloc, qcap);
+    if (r) return r;
+    HIPCHK(hipMemset(c->d_q, 0, qcap * sizeof(Job)));
+    size_t cap = 0;
+    if ((r = dev_grow(c->d_epoch, cap, 1))) return r;
+    HIPCHK(hipMemset(c->d_epoch, 0, sizeof(int32_t)));
+    cap = 0;
+    if ((r = dev_grow(c->d_sc, cap, 1))) return r;
+    *out = c.get();
+    g_ctx[dev] = std::move(c);
+    return DM_OK;
+}
+
+int device_lut(DeviceCtx *c, int K, const uint16_t **out) {
+    const PatternTable *tab = pattern_table(K);
+    if (!tab) return last_code();
+    if (!c->d_lut[K]) {
+        uint16_t *p = nullptr;
+        size_t cap = 0;
+        int r = dev_grow(p, cap, tab->q2c.size());
+        if (r) return r;
+        hipError_t e = hipMemcpy(p, tab->q2c.data(), tab->q2c.size() * 2, hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            (void)hipFree(p);
+            return set_error(DM_ERR_HIP, "class table upload: %s", hipGetErrorString(e));
+        }
+        c->d_lut[K] = p;
+    }
+    *out = c->d_lut[K];
+    return DM_OK;
+}
+
+void free_device_graph(DeviceGraph *d) {
     if (!d) return;
     auto F = [](void *p) {
         if (p) (void)hipFree(p);
@@ -94,138 +158,113 @@ static void device_free(DeviceGraph *d) {
     F(d->d_off);
     F(d->d_adj);
     F(d->d_ostart);
-    F(d->d_arena);
-    F(d->d_q);
-    F(d->d_sq_off);
-    F(d->d_sq_jobs);
-    F(d->d_sq_cur);
-    F(d->d_sc);
-    F(d->d_results);
-    F(d->d_qd);
-    F(d->d_epoch);
     F(d->d_root_cost);
     F(d->d_root_count);
-    for (auto &l : d->d_lut) F(l);
-    F(d->d_mlp);
-    if (d->ev0) (void)hipEventDestroy(d->ev0);
-    if (d->ev1) (void)hipEventDestroy(d->ev1);
-    if (d->stream) (void)hipStreamDestroy(d->stream);
     delete d;
 }
 
-template <class T>
-static int grow(T *&ptr, size_t &cap, size_t need) {
-    return dev_grow(ptr, cap, need);
-}
-
-int ensure_queue(DeviceGraph *d) {
-    const size_t qcap = (size_t)1 << 20;
-    if (d->q_cap_alloc >= qcap && d->d_epoch) return DM_OK;
-    int r = grow(d->d_q, d->q_cap_alloc, qcap);
+int upload_csr(const HostCSR &rc, const std::vector<int64_t> &ostart, DeviceGraph *d) {
+    size_t cap = 0;
+    int r = dev_grow(d->d_off, cap, (size_t)rc.n + 1);
     if (r) return r;
-    HIPCHK(hipMemset(d->d_q, 0, qcap * sizeof(Job)));  // epoch 0 never matches
-    size_t c = 0;
-    if ((r = grow(d->d_epoch, c, 1))) return r;
-    HIPCHK(hipMemset(d->d_epoch, 0, sizeof(int32_t)));
+    cap = 0;
+    if ((r = dev_grow(d->d_adj, cap, std::max<size_t>(rc.adj.size(), 1)))) return r;
+    cap = 0;
+    if ((r = dev_grow(d->d_ostart, cap, std::max<size_t>(ostart.size(), 1)))) return r;
+    HIPCHK(hipMemcpy(d->d_off, rc.off.data(), rc.off.size() * 8, hipMemcpyHostToDevice));
+    if (!rc.adj.empty()) HIPCHK(hipMemcpy(d->d_adj, rc.adj.data(), rc.adj.size() * 4, hipMemcpyHostToDevice));
+    if (!ostart.empty()) HIPCHK(hipMemcpy(d->d_ostart, ostart.data(), ostart.size() * 8, hipMemcpyHostToDevice));
     return DM_OK;
 }
 
 int ensure_device(dm_graph *g) {
-    if (g->dev) return DM_OK;
-    int ndev = 0;
-    hipError_t e = hipGetDeviceCount(&ndev);
-    if (e != hipSuccess || ndev < 1) return set_error(DM_ERR_HIP, "no HIP device available");
-    HIPCHK(hipSetDevice(0));
+    if (g->dev) return hipSetDevice(g->dev->ctx->device) == hipSuccess ? DM_OK : set_error(DM_ERR_HIP, "hipSetDevice");
+    DeviceCtx *c = nullptr;
+    int r = device_ctx(&c);
+    if (r) return r;
     DeviceGraph *d = new DeviceGraph();
+    d->ctx = c;
+    if ((r = upload_csr(g->rcsr, g->ostart, d))) {
+        free_device_graph(d);
+        return r;
+    }
     g->dev = d;
-    hipDeviceProp_t prop;
-    HIPCHK(hipGetDeviceProperties(&prop, 0));
-    d->num_cu = prop.multiProcessorCount;
-    int rate = 0;
-    if (hipDeviceGetAttribute(&rate, hipDeviceAttributeWallClockRate, 0) != hipSuccess || rate <= 0) rate = 100000;
-    d->wall_khz = rate;
-    snprintf(d->arch, sizeof(d->arch), "%s", prop.gcnArchName);
-    HIPCHK(hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking));
-    HIPCHK(hipEventCreate(&d->ev0));
-    HIPCHK(hipEventCreate(&d->ev1));
-    const HostCSR &rc = g->rcsr;
-    size_t cap = 0;
-    int rc_ = grow(d->d_off, cap, (size_t)rc.n + 1);
-    if (rc_) return rc_;
-    cap = 0;
-    if ((rc_ = grow(d->d_adj, cap, std::max<size_t>(rc.adj.size(), 1)))) return rc_;
-    cap = 0;
-    if ((rc_ = grow(d->d_ostart, cap, std::max<size_t>(g->ostart.size(), 1)))) return rc_;
-    HIPCHK(hipMemcpy(d->d_off, rc.off.data(), rc.off.size() * 8, hipMemcpyHostToDevice));
-    if (!rc.adj.empty()) HIPCHK(hipMemcpy(d->d_adj, rc.adj.data(), rc.adj.size() * 4, hipMemcpyHostToDevice));
-    if (!g->ostart.empty())
-        HIPCHK(hipMemcpy(d->d_ostart, g->ostart.data(), g->ostart.size() * 4, hipMemcpyHostToDevice));
-    size_t c1 = 0;
-    if ((rc_ = grow(d->d_sc, c1, 1))) return rc_;
     return DM_OK;
 }
 
 // ---------------------------------------------------------------------------
 // launch shape
-static int max_lds_uncached(int vwave, int mode, int K, int ncls_single, int hist_slots);
+int block_threads(const dm_config &cfg, int *threads) {
+    const int w = cfg.waves_per_block == 0 ? 4 : cfg.waves_per_block;
+    if (w != 1 && w != 2 && w != 4) return set_error(DM_ERR_ARG, "waves_per_block must be 1, 2 or 4 (got %d)", w);
+    *threads = 64 * w;
+    return DM_OK;
+}
+
+static int max_lds_uncached(int vwave, int threads, int mode, int K, int ncls_single, int hist_slots);
 
 // Largest per-worker LDS stack (a multiple of 256 B, at most 16 KiB) that
 // keeps the occupancy the kernel's registers allow: the LDS of a CU (160
 // KiB) divided over that many blocks, less each block's fixed records.
-int max_lds_per_worker(int vwave, int mode, int K, int ncls_single, int hist_slots) {
+int max_lds_per_worker(int vwave, int threads, int mode, int K, int ncls_single, int hist_slots) {
     // the answer depends only on the arguments (and the one device): memoised
     static std::mutex mu;
-    static std::map<std::tuple<int, int, int, int, int>, int> memo;
-    const auto key = std::make_tuple(vwave, mode, K, ncls_single, hist_slots);
+    static std::map<std::tuple<int, int, int, int, int, int>, int> memo;
+    const auto key = std::make_tuple(vwave, threads, mode, K, ncls_single, hist_slots);
     {
         std::lock_guard<std::mutex> lk(mu);
         auto it = memo.find(key);
         if (it != memo.end()) return it->second;
     }
-    const int lds = max_lds_uncached(vwave, mode, K, ncls_single, hist_slots);
+    const int lds = max_lds_uncached(vwave, threads, mode, K, ncls_single, hist_slots);
     std::lock_guard<std::mutex> lk(mu);
     memo[key] = lds;
     return lds;
 }
 
-static int max_lds_uncached(int vwave, int mode, int K, int ncls_single, int hist_slots) {
+static int max_lds_uncached(int vwave, int threads, int mode, int K, int ncls_single, int hist_slots) {
+    const int wpb = threads / vwave;
     int bpc = 0;
-    const size_t fixed = enum_lds_bytes(vwave, 256, K, ncls_single, hist_slots, 0);
-    if (enum_occupancy(vwave, mode, 256, fixed, &bpc) != DM_OK || bpc < 1) return 32768 / (256 / vwave);
+    const size_t fixed = enum_lds_bytes(vwave, threads, K, ncls_single, hist_slots, 0);
+    if (enum_occupancy(vwave, mode, threads, fixed, &bpc) != DM_OK || bpc < 1) return 32768 / wpb;
     const size_t per_block = (size_t)160 * 1024 / (size_t)bpc;
-    if (per_block <= fixed + 256) return 32768 / (256 / vwave);
-    int lds = std::min((int)(((per_block - fixed) / (size_t)(256 / vwave)) & ~(size_t)255), 16384);
+    if (per_block <= fixed + 256) return std::min(32768 / wpb, 16384);
+    int lds = std::min((int)(((per_block - fixed) / (size_t)wpb) & ~(size_t)255), 16384);
     // the LDS allocation granularity may cost a block: step down until the
     // occupancy holds
     for (; lds > 256; lds -= 256) {
         int b2 = 0;
-        const size_t total = enum_lds_bytes(vwave, 256, K, ncls_single, hist_slots, lds);
-        if (enum_occupancy(vwave, mode, 256, total, &b2) == DM_OK && b2 >= bpc) break;
+        const size_t total = enum_lds_bytes(vwave, threads, K, ncls_single, hist_slots, lds);
+        if (enum_occupancy(vwave, mode, threads, total, &b2) == DM_OK && b2 >= bpc) break;
     }
     return lds;
 }
 
 int pick_shape(dm_graph *g, int K, bool motif, const dm_config &cfg, Shape &sh) {
-    DeviceGraph *d = g->dev;
+    DeviceCtx *c = g->dev->ctx;
     int vw = cfg.vwave;
     if (vw == 0) vw = autotuned_vwave(g, K, motif);
     if (vw != 8 && vw != 16 && vw != 32 && vw != 64)
         return set_error(DM_ERR_ARG, "vwave must be 8, 16, 32 or 64 (got %d)", vw);
+    int threads = 256;
+    int r = block_threads(cfg, &threads);
+    if (r) return r;
     int ncls = 0;
     if (motif) ncls = pattern_table(K)->ncls;
-    int lds = cfg.lds_bytes > 0 ? cfg.lds_bytes
-                                : tuned_lds(g, K, motif, vw, max_lds_per_worker(vw, motif ? 1 : 0, K, ncls, 0));
+    // a tuned LDS size belongs to the full-size block it was tuned with
+    const int dl = max_lds_per_worker(vw, threads, motif ? 1 : 0, K, ncls, 0);
+    int lds = cfg.lds_bytes > 0 ? cfg.lds_bytes : threads == 256 ? tuned_lds(g, K, motif, vw, dl) : dl;
     lds = (lds + 15) & ~15;
-    size_t lds_total = enum_lds_bytes(vw, 256, K, ncls, 0, lds);
+    size_t lds_total = enum_lds_bytes(vw, threads, K, ncls, 0, lds);
     int bpc = 0;
-    int r = enum_occupancy(vw, motif ? 1 : 0, 256, lds_total, &bpc);
-    if (r) return r;
+    if ((r = enum_occupancy(vw, motif ? 1 : 0, threads, lds_total, &bpc))) return r;
     if (bpc < 1) return set_error(DM_ERR_ARG, "launch does not fit one CU (LDS %zu B)", lds_total);
     sh.vwave = vw;
+    sh.threads = threads;
     sh.lds_per_worker = lds;
     sh.lds_total = lds_total;
-    sh.blocks = cfg.blocks > 0 ? cfg.blocks : bpc * d->num_cu;
-    sh.workers = sh.blocks * (256 / vw);
+    sh.blocks = cfg.blocks > 0 ? cfg.blocks : bpc * c->num_cu;
+    sh.workers = sh.blocks * (threads / vw);
     return DM_OK;
 }
 
@@ -251,6 +290,19 @@ int64_t arena_bound_for(int32_t maxdeg, int32_t max_out_, int K, bool motif) {
 
 static int64_t arena_bound(const dm_graph *g, int K, bool motif) {
     return arena_bound_for(g->rcsr.maxdeg, g->max_out, K, motif);
+}
+
+int device_error(int code, const char *what) {
+    const char *why = code == 1 ? "watchdog expired" : code == 2 ? "stack arena overflow"
+                    : code == 4 ? "stopped" : "invalid job";
+    return set_error(DM_ERR_DEVICE, "%s aborted: %s", what, why);
+}
+
+int check_config(const dm_config &cfg) {
+    if (cfg.steal_min == 1 || cfg.steal_min < 0 || cfg.steal_period < 0)
+        return set_error(DM_ERR_ARG, "steal_min must be 0 (tuned) or >= 2; steal_period >= 0");
+    int t = 0;
+    return block_threads(cfg, &t);
 }
 
 // ---------------------------------------------------------------------------
@@ -338,11 +390,12 @@ int run_enum(dm_graph *g, int K, bool motif, const dm_config &cfg, uint64_t *his
     } else if (K < 2 || K > kMaxK) {
         return set_error(DM_ERR_ARG, "clique size k=%d unsupported (2..%d)", K, kMaxK);
     }
-    if (cfg.steal_min == 1 || cfg.steal_min < 0 || cfg.steal_period < 0)
-        return set_error(DM_ERR_ARG, "steal_min must be 0 (tuned) or >= 2; steal_period >= 0");
-    int r = ensure_device(g);
+    int r = check_config(cfg);
     if (r) return r;
-    DeviceGraph *d = g->dev;
+    if ((r = ensure_device(g))) return r;
+    DeviceGraph *dg = g->dev;
+    DeviceCtx *d = dg->ctx;
+    std::lock_guard<std::recursive_mutex> lock(d->mu);
     const int32_t n = g->rcsr.n;
 
     Shape sh;
@@ -395,44 +448,40 @@ int run_enum(dm_graph *g, int K, bool motif, const dm_config &cfg, uint64_t *his
 
     // device workspaces
     size_t need_arena = (size_t)arena_bound(g, K, motif) * (size_t)sh.workers;
-    if ((r = grow(d->d_arena, d->arena_cap, need_arena))) return r;
-    if ((r = grow(d->d_sq_off, d->sq_off_cap, sq_off.size()))) return r;
-    if ((r = grow(d->d_sq_jobs, d->sq_jobs_cap, std::max<size_t>(sq_jobs.size(), 4)))) return r;
-    if ((r = grow(d->d_sq_cur, d->sq_cur_cap, (size_t)sh.workers))) return r;
-    if ((r = ensure_queue(d))) return r;
+    if ((r = dev_grow(d->d_arena, d->arena_cap, need_arena))) return r;
+    if ((r = dev_grow(d->d_sq_off, d->sq_off_cap, sq_off.size()))) return r;
+    if ((r = dev_grow(d->d_sq_jobs, d->sq_jobs_cap, std::max<size_t>(sq_jobs.size(), 4)))) return r;
+    if ((r = dev_grow(d->d_sq_cur, d->sq_cur_cap, (size_t)sh.workers))) return r;
     const size_t ncls = tab ? (size_t)tab->ncls : 0;
     const size_t nres = motif ? ncls : 1;
-    if ((r = grow(d->d_results, d->results_cap, std::max<size_t>(nres, 1)))) return r;
-    if (tab && !d->d_lut[K]) {
-        size_t c = 0;
-        if ((r = grow(d->d_lut[K], c, tab->q2c.size()))) return r;
-        HIPCHK(hipMemcpy(d->d_lut[K], tab->q2c.data(), tab->q2c.size() * 2, hipMemcpyHostToDevice));
-    }
+    if ((r = dev_grow(d->d_results, d->results_cap, std::max<size_t>(nres, 1)))) return r;
+    const uint16_t *lut = nullptr;
+    if (tab && (r = device_lut(d, K, &lut))) return r;
     const bool want_cost = cfg.record_costs != 0, want_cnt = cfg.per_root != 0;
-    if (want_cost && (r = grow(d->d_root_cost, d->root_cost_cap, (size_t)n))) return r;
-    if (want_cnt && (r = grow(d->d_root_count, d->root_count_cap, (size_t)n))) return r;
+    if (want_cost && (r = dev_grow(dg->d_root_cost, dg->root_cost_cap, (size_t)n))) return r;
+    if (want_cnt && (r = dev_grow(dg->d_root_count, dg->root_count_cap, (size_t)n))) return r;
     QueryDesc qd;
     memset(&qd, 0, sizeof(qd));
     qd.K = K;
     qd.motif = motif ? 1 : 0;
     qd.ncls = (int32_t)ncls;
-    qd.lut = tab ? d->d_lut[K] : nullptr;
+    qd.lut = lut;
     qd.out = d->d_results;
-    if ((r = grow(d->d_qd, d->qd_cap, 1))) return r;
+    if ((r = dev_grow(d->d_qd, d->qd_cap, 1))) return r;
 
     hipStream_t s = d->stream;
     HIPCHK(hipMemcpyAsync(d->d_qd, &qd, sizeof(qd), hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync(d->d_sq_off, sq_off.data(), sq_off.size() * 8, hipMemcpyHostToDevice, s));
     if (!sq_jobs.empty())
         HIPCHK(hipMemcpyAsync(d->d_sq_jobs, sq_jobs.data(), sq_jobs.size() * 4, hipMemcpyHostToDevice, s));
-    if (want_cost && (r = fill_u32_launch(d->d_root_cost, 0, (size_t)n * 2, s))) return r;
-    if (want_cnt && (r = fill_u32_launch(d->d_root_count, 0, (size_t)n * 2, s))) return r;
+    if (want_cost && (r = fill_u32_launch(dg->d_root_cost, 0, (size_t)n * 2, s))) return r;
+    if (want_cnt && (r = fill_u32_launch(dg->d_root_count, 0, (size_t)n * 2, s))) return r;
 
     EnumParams p;
     memset(&p, 0, sizeof(p));
-    p.off = d->d_off;
-    p.adj = d->d_adj;
-    p.ostart = d->d_ostart;
+    p.off = dg->d_off;
+    p.adj = dg->d_adj;
+    p.ostart = dg->d_ostart;
     p.n = n;
     p.nq = 1;
     p.qd = d->d_qd;
@@ -450,8 +499,8 @@ int run_enum(dm_graph *g, int K, bool motif, const dm_config &cfg, uint64_t *his
     p.arena = (char *)d->d_arena;
     p.arena_per_worker = arena_bound(g, K, motif);
     p.lds_per_worker = sh.lds_per_worker;
-    p.root_cost = want_cost ? d->d_root_cost : nullptr;
-    p.root_count = want_cnt ? d->d_root_count : nullptr;
+    p.root_cost = want_cost ? dg->d_root_cost : nullptr;
+    p.root_count = want_cnt ? dg->d_root_count : nullptr;
     p.steal = cfg.steal ? 1 : 0;
     p.steal_min = cfg.steal_min ? cfg.steal_min : tuned_steal_min(g, K, motif);
     p.poll_ticks = (uint64_t)cfg.steal_period * (uint64_t)d->wall_khz / 1000;
@@ -463,7 +512,7 @@ int run_enum(dm_graph *g, int K, bool motif, const dm_config &cfg, uint64_t *his
         return r;
 
     HIPCHK(hipEventRecord(d->ev0, s));
-    if (njobs > 0 && (r = enum_launch(p, motif ? 1 : 0, sh.vwave, sh.blocks, 256, sh.lds_total, s))) return r;
+    if (njobs > 0 && (r = enum_launch(p, motif ? 1 : 0, sh.vwave, sh.blocks, sh.threads, sh.lds_total, s))) return r;
     HIPCHK(hipEventRecord(d->ev1, s));
     SchedCounters sc1;
     HIPCHK(hipMemcpyAsync(&sc1, d->d_sc, sizeof(sc1), hipMemcpyDeviceToHost, s));
@@ -472,10 +521,7 @@ int run_enum(dm_graph *g, int K, bool motif, const dm_config &cfg, uint64_t *his
     HIPCHK(hipStreamSynchronize(s));
     float kms = 0;
     HIPCHK(hipEventElapsedTime(&kms, d->ev0, d->ev1));
-    if (sc1.error) {
-        const char *why = sc1.error == 1 ? "watchdog expired" : sc1.error == 2 ? "stack arena overflow" : "invalid job";
-        return set_error(DM_ERR_DEVICE, "enumeration kernel aborted: %s", why);
-    }
+    if (sc1.error) return device_error(sc1.error, "enumeration kernel");
     uint64_t total = 0;
     if (motif) {
         for (size_t i = 0; i < ncls; i++) total += h[i];
@@ -489,7 +535,7 @@ int run_enum(dm_graph *g, int K, bool motif, const dm_config &cfg, uint64_t *his
     // per-root records back in original ids
     if (want_cost) {
         std::vector<unsigned long long> tmp(n);
-        HIPCHK(hipMemcpy(tmp.data(), d->d_root_cost, (size_t)n * 8, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(tmp.data(), dg->d_root_cost, (size_t)n * 8, hipMemcpyDeviceToHost));
         g->last_costs.assign(n, 0);
         for (int32_t v = 0; v < n; v++) g->last_costs[g->inv[v]] = tmp[v];
     } else {
@@ -497,7 +543,7 @@ int run_enum(dm_graph *g, int K, bool motif, const dm_config &cfg, uint64_t *his
     }
     if (want_cnt) {
         std::vector<unsigned long long> tmp(n);
-        HIPCHK(hipMemcpy(tmp.data(), d->d_root_count, (size_t)n * 8, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(tmp.data(), dg->d_root_count, (size_t)n * 8, hipMemcpyDeviceToHost));
         g->last_counts.assign(n, 0);
         for (int32_t v = 0; v < n; v++) g->last_counts[g->inv[v]] = tmp[v];
     } else {
@@ -540,7 +586,7 @@ using namespace dm;
 // C ABI
 extern "C" {
 
-int dm_version(void) { return 1; }
+int dm_version(void) { return 2; }
 
 const char *dm_last_error(void) { return t_msg; }
 
@@ -611,7 +657,7 @@ int dm_graph_csr(const dm_graph *g, int64_t *off, size_t off_len, int32_t *adj, 
 
 void dm_graph_free(dm_graph *g) {
     if (!g) return;
-    device_free(g->dev);
+    free_device_graph(g->dev);
     free_model(g);
     delete g;
 }

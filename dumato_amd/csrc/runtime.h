@@ -1,9 +1,11 @@
-// Host-side state of a dm_graph handle.
+// Host-side state of the runtime: one context per GPU, shared by every graph
+// handle and batch on it, and the per-handle graph data.
 #ifndef DM_RUNTIME_H
 #define DM_RUNTIME_H
 
 #include <hip/hip_runtime_api.h>
 
+#include <mutex>
 #include <vector>
 
 #include "internal.h"
@@ -11,20 +13,29 @@
 
 namespace dm {
 
-struct DeviceGraph {
+// State of one GPU, created on first use and kept until the process ends.
+// Every counting launch on the device goes to its one stream, so launches run
+// one after another on the GPU; that is what lets them share the donated-job
+// queue (slots are tagged with the launch epoch) and the workspaces.  `mu` is
+// held for the whole of a call that uses the stream or the workspaces (it is
+// recursive: a learned partition runs a k=3 count inside a counting call).
+struct DeviceCtx {
+    int device = 0;
     int num_cu = 0;
     int wall_khz = 100000;
-    char arch[64] = "";
+    char arch[256] = "";
+    std::recursive_mutex mu;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    int64_t *d_off = nullptr;
-    int32_t *d_adj = nullptr;
-    int32_t *d_ostart = nullptr;
-    // grow-only workspaces
-    char *d_arena = nullptr;
-    size_t arena_cap = 0;
+    // donated-job queue (fixed size, never reallocated: batches capture it)
     Job *d_q = nullptr;
     size_t q_cap_alloc = 0;
+    int32_t *d_epoch = nullptr;
+    // class tables per k (uploaded once, never reallocated)
+    uint16_t *d_lut[kMaxMotifK + 1] = {nullptr};
+    // grow-only workspaces of single-graph calls
+    char *d_arena = nullptr;
+    size_t arena_cap = 0;
     int64_t *d_sq_off = nullptr;
     size_t sq_off_cap = 0;
     int32_t *d_sq_jobs = nullptr;
@@ -36,18 +47,27 @@ struct DeviceGraph {
     size_t results_cap = 0;
     QueryDesc *d_qd = nullptr;
     size_t qd_cap = 0;
-    int32_t *d_epoch = nullptr;
+};
+
+// The current device's context (created, with its stream, events and queue,
+// on the first call).
+int device_ctx(DeviceCtx **out);
+
+// Device copy of one graph handle.
+struct DeviceGraph {
+    DeviceCtx *ctx = nullptr;
+    int64_t *d_off = nullptr;
+    int32_t *d_adj = nullptr;
+    int64_t *d_ostart = nullptr;
     unsigned long long *d_root_cost = nullptr;
     size_t root_cost_cap = 0;
     unsigned long long *d_root_count = nullptr;
     size_t root_count_cap = 0;
-    uint16_t *d_lut[kMaxMotifK + 1] = {nullptr};
-    float *d_mlp = nullptr;  // MLP workspace (sched.cpp)
-    size_t mlp_cap = 0;
 };
 
 struct Shape {
     int vwave = 64;
+    int threads = 256;
     int lds_per_worker = 0;
     size_t lds_total = 0;
     int blocks = 0;
@@ -60,7 +80,7 @@ struct dm_graph {
     dm::HostCSR csr;                 // original ids
     dm::HostCSR rcsr;                // degree-rank ids (ascending (degree, id))
     std::vector<int32_t> perm, inv;  // perm[orig] = rank, inv[rank] = orig
-    std::vector<int32_t> ostart;     // per rank: index of the first neighbour above it
+    std::vector<int64_t> ostart;     // per rank: index of the first neighbour above it
     int32_t max_out = 0;
     dm::DeviceGraph *dev = nullptr;
     dm_stats last{};
@@ -101,8 +121,15 @@ inline int dev_grow(T *&ptr, size_t &cap, size_t need) {
 }
 int64_t arena_bound_for(int32_t maxdeg, int32_t max_out, int K, bool motif);
 int graph_finish(dm_graph *g);
+// Uploads g's CSR to the current device (once) and binds g to its context.
 int ensure_device(dm_graph *g);
-int ensure_queue(DeviceGraph *d);
+// Uploads a device copy of a degree-rank CSR (off, adj, ostart).
+int upload_csr(const HostCSR &rc, const std::vector<int64_t> &ostart, DeviceGraph *d);
+void free_device_graph(DeviceGraph *d);
+// The class table of k on the context's device (uploaded on first use).
+int device_lut(DeviceCtx *c, int K, const uint16_t **out);
+// Threads per block for cfg.waves_per_block (1, 2 or 4 waves; 0 = 4).
+int block_threads(const dm_config &cfg, int *threads);
 int pick_shape(dm_graph *g, int K, bool motif, const dm_config &cfg, Shape &sh);
 bool root_has_work(const dm_graph *g, int32_t v, int K, bool motif);
 void plain_partition(const dm_graph *g, const std::vector<int32_t> &roots, int K, bool motif, int workers,
@@ -121,9 +148,12 @@ void write_queues(const std::vector<StaticJob> &jobs, const std::vector<std::vec
                   std::vector<int64_t> &sq_off, std::vector<int32_t> &sq_jobs);
 int run_enum(dm_graph *g, int K, bool motif, const dm_config &cfg, uint64_t *hist, size_t hist_len,
              uint64_t *count_out, int sample_mod = 1);
+int check_config(const dm_config &cfg);
 int tuned_steal_min(const dm_graph *g, int K, bool motif);
 int tuned_lds(const dm_graph *g, int K, bool motif, int vwave, int fallback);
-int max_lds_per_worker(int vwave, int mode, int K, int ncls_single, int hist_slots);
+int max_lds_per_worker(int vwave, int threads, int mode, int K, int ncls_single, int hist_slots);
+// Kernel abort reason of a launch's error word as a DM_ERR_DEVICE failure.
+int device_error(int code, const char *what);
 
 // sched.cpp
 int autotuned_vwave(dm_graph *g, int K, bool motif);

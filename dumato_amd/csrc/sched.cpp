@@ -98,7 +98,7 @@ int autotune(dm_graph *g, int K, bool motif, double budget_ms, dm_config *best_o
         }
     }
     const int ncls = motif ? pattern_table(K)->ncls : 0;
-    const int dl = max_lds_per_worker(bc.vwave, motif ? 1 : 0, K, ncls, 0);
+    const int dl = max_lds_per_worker(bc.vwave, 256, motif ? 1 : 0, K, ncls, 0);
     if (full_search) {
         const int d8 = 32768 / (256 / bc.vwave);  // 8 KiB for a full wave
         for (int lds : {(dl / 2) & ~255, std::min(d8, dl), dl}) {
@@ -363,6 +363,8 @@ static int train_model(const std::vector<float> &X, const std::vector<float> &Y,
 static int predict_all(dm_graph *g, std::vector<float> &pred) {
     CostModel &M = *g->model;
     const int32_t n = g->rcsr.n;
+    DeviceCtx *ctx = g->dev->ctx;
+    std::lock_guard<std::recursive_mutex> lock(ctx->mu);
     if (!M.d_params) {
         std::vector<float> X((size_t)n * kMlpIn);
         for (int32_t v = 0; v < n; v++)
@@ -374,11 +376,11 @@ static int predict_all(dm_graph *g, std::vector<float> &pred) {
         HIPCHK(hipMemcpy(M.d_params, M.params.data(), kMlpParams * 4, hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(M.d_x, X.data(), X.size() * 4, hipMemcpyHostToDevice));
     }
-    int r = mlp_forward_launch(M.d_params, M.d_x, n, M.d_out, g->dev->stream);
+    int r = mlp_forward_launch(M.d_params, M.d_x, n, M.d_out, ctx->stream);
     if (r) return r;
     pred.resize(n);
-    HIPCHK(hipMemcpyAsync(pred.data(), M.d_out, (size_t)n * 4, hipMemcpyDeviceToHost, g->dev->stream));
-    HIPCHK(hipStreamSynchronize(g->dev->stream));
+    HIPCHK(hipMemcpyAsync(pred.data(), M.d_out, (size_t)n * 4, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
     for (auto &p : pred) p = p * M.y_std + M.y_mean;
     return DM_OK;
 }

@@ -47,7 +47,7 @@ typedef struct dm_config {
     int32_t steal_period;    /* microseconds between a worker's checks for idle
                                 workers (0: every DFS step) */
     int32_t lds_bytes;       /* LDS stack bytes per worker; 0 = autotuned */
-    int32_t waves_per_block; /* 0 = 4 */
+    int32_t waves_per_block; /* wave64s per block: 1, 2 or 4 (0 = 4) */
     int32_t blocks;          /* 0 = as many as stay resident on the GPU */
     int32_t record_costs;    /* 1: per-root cycle counts (dm_last_root_costs) */
     int32_t per_root;        /* 1: per-root subgraph counts (dm_last_root_counts) */
@@ -113,9 +113,12 @@ int dm_gen_barabasi_albert(int32_t n, int32_t m_per_vertex,
  * outputAggregateCounter, src/main/DuMatoCPU.cu:316-362,419-423). */
 int dm_clique_count(dm_graph *g, int k, const dm_config *cfg, uint64_t *out);
 
-/* Histogram of connected induced k-vertex subgraphs, 3 <= k <= 6, slot i =
+/* Histogram of connected induced k-vertex subgraphs, 3 <= k <= 7, slot i =
  * the reference's class id cgLocal i (dictionaries/k.csv); hist_len must be
- * dm_motif_num_classes(k).  Replaces motifs' kernel + host loop
+ * dm_motif_num_classes(k).  k = 7 has no reference dictionary: its 1043 slots
+ * are this library's own class ids (isomorphism classes ranked by their
+ * largest quick code), so only its per-class counts, not its ids, match the
+ * reference's semantics.  Replaces motifs' kernel + host loop
  * (src/main/motifs_DM_WCLB.cu:9-36, DuMatoCPU::outputAggregatePattern,
  * src/main/DuMatoCPU.cu:364-384).  *total (nullable) = sum of hist. */
 int dm_motif_count(dm_graph *g, int k, const dm_config *cfg, uint64_t *hist,
@@ -128,7 +131,7 @@ int dm_last_root_costs(const dm_graph *g, uint64_t *cycles, size_t n);
 int dm_last_root_counts(const dm_graph *g, uint64_t *counts, size_t n);
 
 /* ---- pattern tables (reference: QuickMapping, src/main/QuickMapping.cpp:5-43) ---- */
-/* Number of classes for k (3, 10, 33, 155 for k = 3..6). */
+/* Number of classes for k (3, 10, 33, 155, 1043 for k = 3..7). */
 int dm_motif_num_classes(int k, size_t *ncls);
 /* Regenerated tables: quick code -> class id (2^((k+1)(k-2)/2) entries) and
  * class id -> canonical lower-triangle code (the reference's cg_global). */

@@ -168,9 +168,19 @@ def clique_count_roots(g, k, roots, nthreads=0, faithful=False):
     return int(r), pr[:len(roots)]
 
 
-def motif_count(g, k, nthreads=0, faithful=False, per_root=False):
+def _table(k, table):
+    """(lut, ncls): the reference dictionary of k, or a caller-given
+    (quick -> class) table with its class count (k = 7 has no reference
+    dictionary)."""
+    if table is None:
+        return load_lut(k)
+    lut, ncls = table
+    return np.ascontiguousarray(lut, dtype=np.uint16), int(ncls)
+
+
+def motif_count(g, k, nthreads=0, faithful=False, per_root=False, table=None):
     L = lib()
-    lut, ncls = load_lut(k)
+    lut, ncls = _table(k, table)
     hist = np.zeros(ncls, dtype=np.uint64)
     pr = np.zeros(max(g.n, 1), dtype=np.uint64) if per_root else None
     r = L.or_motif_count(g.n, _ptr(g.off, i64p), _ptr(g.adj, i32p), k, _ptr(lut, u16p), ncls,
@@ -181,9 +191,9 @@ def motif_count(g, k, nthreads=0, faithful=False, per_root=False):
     return (int(r), hist, pr[:g.n]) if per_root else (int(r), hist)
 
 
-def motif_count_roots(g, k, roots, nthreads=0, faithful=False):
+def motif_count_roots(g, k, roots, nthreads=0, faithful=False, table=None):
     L = lib()
-    lut, ncls = load_lut(k)
+    lut, ncls = _table(k, table)
     roots = np.ascontiguousarray(roots, dtype=np.int32)
     hist = np.zeros(ncls, dtype=np.uint64)
     pr = np.zeros(max(len(roots), 1), dtype=np.uint64)
@@ -197,8 +207,8 @@ def brute_clique(g, k):
     return int(lib().or_brute_clique(g.n, _ptr(g.off, i64p), _ptr(g.adj, i32p), k))
 
 
-def brute_motif(g, k):
-    lut, ncls = load_lut(k)
+def brute_motif(g, k, table=None):
+    lut, ncls = _table(k, table)
     hist = np.zeros(ncls, dtype=np.uint64)
     t = lib().or_brute_motif(g.n, _ptr(g.off, i64p), _ptr(g.adj, i32p), k, _ptr(lut, u16p), ncls,
                              _ptr(hist, u64p))
