@@ -81,6 +81,8 @@ SYMBOLS = [
                                               ctypes.c_int32, ctypes.c_uint64, ctypes.POINTER(GP)]),
     ("dm_clique_count", ctypes.c_int, [GP, ctypes.c_int, ctypes.POINTER(Config), u64p]),
     ("dm_motif_count", ctypes.c_int, [GP, ctypes.c_int, ctypes.POINTER(Config), u64p, ctypes.c_size_t, u64p]),
+    ("dm_count_noalloc", ctypes.c_int, [GP, ctypes.c_int, ctypes.c_int, ctypes.POINTER(Config), u64p,
+                                        ctypes.c_size_t, u64p]),
     ("dm_last_stats", ctypes.c_int, [GP, ctypes.POINTER(Stats)]),
     ("dm_last_root_costs", ctypes.c_int, [GP, u64p, ctypes.c_size_t]),
     ("dm_last_root_counts", ctypes.c_int, [GP, u64p, ctypes.c_size_t]),
@@ -222,6 +224,16 @@ class Graph:
         _check(lib().dm_motif_count(self._h, int(k), ctypes.byref(cfg) if cfg else None,
                                     _ptr(hist, u64p), ncls, ctypes.byref(tot)))
         return int(tot.value), hist
+
+    def count_noalloc(self, motif, k, cfg=None):
+        """dm_count_noalloc: (total, counts) -- counts = [clique count] or the
+        motif histogram -- computed without device allocation."""
+        n = num_classes(k) if motif else 1
+        out = np.zeros(n, np.uint64)
+        tot = ctypes.c_uint64()
+        _check(lib().dm_count_noalloc(self._h, 1 if motif else 0, int(k), ctypes.byref(cfg) if cfg else None,
+                                      _ptr(out, u64p), n, ctypes.byref(tot)))
+        return int(tot.value), out
 
     def last_stats(self):
         s = Stats()

@@ -124,6 +124,23 @@ int dm_clique_count(dm_graph *g, int k, const dm_config *cfg, uint64_t *out);
 int dm_motif_count(dm_graph *g, int k, const dm_config *cfg, uint64_t *hist,
                    size_t hist_len, uint64_t *total);
 
+/* Counting that allocates no device memory.  No counterpart in the
+ * reference: it is for callers that cannot allocate (code under stream
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * results as dm_clique_count (motif = 0: out[0]) and dm_motif_count
+ * (motif = 1: out[0 .. dm_motif_num_classes(k)), *total = their sum).
+ * Every device buffer beyond the graph and the result comes from one static
+ * device array of DM_NOALLOC_SCRATCH_BYTES per GPU, which keeps the last
+ * call's contents; calls on one GPU are serialised by the library, and other
+ * work on that array while a call runs would race with it.  Limits: the
+ * graph must already be on the GPU (its first counting or tuning call
+ * uploads it); plain scheduler only, no per-root records or root ranges;
+ * the worker count is capped so that the stack arenas fit, and a graph whose
+ * arena for one block exceeds the array fails with DM_ERR_ARG. */
+#define DM_NOALLOC_SCRATCH_BYTES (64u << 20)
+int dm_count_noalloc(dm_graph *g, int motif, int k, const dm_config *cfg, uint64_t *out,
+                     size_t out_len, uint64_t *total);
+
 /* Statistics and optional per-root records of the last counting call on g.
  * Root arrays are indexed by original vertex id (n entries). */
 int dm_last_stats(const dm_graph *g, dm_stats *st);

@@ -140,6 +140,21 @@ def test_generators_deterministic_and_sized():
     assert O.clique_count(ref, 8) >= 10
 
 
+def test_noalloc_limits_checked_on_the_host():
+    # rejected before any device work: sizes beyond the static array and
+    # options that would need allocation
+    s, d, n = G.star(1 << 20)
+    hub = dm.Graph.from_edges(s, d, n)
+    with pytest.raises(dm.DMError) as e:
+        hub.count_noalloc(True, 6)
+    assert e.value.code == -1 and "static scratch" in str(e.value)
+    g = dm.Graph.from_edges(*G.complete(5)[:2])
+    for bad in (dict(sched=dm.DM_SCHED_LEARNED), dict(record_costs=1), dict(root_end=3)):
+        with pytest.raises(dm.DMError) as e:
+            g.count_noalloc(False, 3, dm.config(**bad))
+        assert e.value.code == -1
+
+
 def test_counting_without_gpu_fails_loudly():
     import ctypes
     L = dm.lib()
