@@ -68,6 +68,15 @@ def test_ba_planted_motif5_root_ranges():
     g = W.ba_100k_planted()
     gold = GOLD["ba_100k_planted"]
     check_graph(g, gold)
+    # the ranges cover the hubs (ids 0..10) and every member of one planted
+    # clique; the members are pairwise adjacent
+    off, adj = g.csr()
+    members = gold["planted_members"]
+    assert len(members) >= 8 and "0-11" in gold["motif5_root_ranges"]
+    for i, a in enumerate(members):
+        nb = set(adj[off[a]:off[a + 1]].tolist())
+        assert all(b in nb for b in members[i + 1:])
+        assert "%d-%d" % (a, a + 1) in gold["motif5_root_ranges"]
     for rng, want in gold["motif5_root_ranges"].items():
         a, b = map(int, rng.split("-"))
         print("case ba motif k=5 roots %s" % rng, flush=True)
