@@ -34,7 +34,7 @@ class Config(ctypes.Structure):
                 ("blocks", ctypes.c_int32), ("record_costs", ctypes.c_int32),
                 ("per_root", ctypes.c_int32), ("root_begin", ctypes.c_int32),
                 ("root_end", ctypes.c_int32), ("clique_bitmap", ctypes.c_int32),
-                ("reserved", ctypes.c_int32), ("watchdog_s", ctypes.c_double)]
+                ("repartition", ctypes.c_int32), ("watchdog_s", ctypes.c_double)]
 
 
 class Stats(ctypes.Structure):
@@ -47,11 +47,32 @@ class Stats(ctypes.Structure):
                 ("blocks", ctypes.c_int32), ("lds_bytes", ctypes.c_int32),
                 ("first_job_ms", ctypes.c_double), ("static_done_ms", ctypes.c_double),
                 ("last_job_ms", ctypes.c_double), ("exit_ms", ctypes.c_double),
-                ("job_max_cycles", ctypes.c_uint64)]
+                ("job_max_cycles", ctypes.c_uint64), ("infer_ms", ctypes.c_double), ("bin_ms", ctypes.c_double)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
 
+
+class DfsConfig(ctypes.Structure):
+    """Mirror of struct dm_dfs_config."""
+    _fields_ = [("vwave", ctypes.c_int32), ("blocks", ctypes.c_int32), ("jobs_per_worker", ctypes.c_int32),
+                ("threshold", ctypes.c_int32), ("poll_us", ctypes.c_int32), ("stop_every_us", ctypes.c_int32),
+                ("max_rebalances", ctypes.c_int32), ("reserved", ctypes.c_int32), ("watchdog_s", ctypes.c_double)]
+
+
+class DfsStats(ctypes.Structure):
+    """Mirror of struct dm_dfs_stats."""
+    _fields_ = [("total_ms", ctypes.c_double), ("lb_ms", ctypes.c_double), ("launches", ctypes.c_int32),
+                ("rebalances", ctypes.c_int32), ("workers", ctypes.c_int32), ("vwave", ctypes.c_int32),
+                ("donated_jobs", ctypes.c_uint64), ("subgraphs", ctypes.c_uint64)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+DM_DFS_CLIQUE_PA = 0
+DM_DFS_CLIQUE_WC = 1
+DM_DFS_MOTIF = 2
 
 _lib = None
 
@@ -102,6 +123,11 @@ SYMBOLS = [
     ("dm_batch_free", None, [GP]),
     ("dm_mlp_grad", ctypes.c_int, [f32p, f32p, f32p, ctypes.c_int, f32p, f32p, f32p]),
     ("dm_mlp_adam", ctypes.c_int, [f32p, f32p, f32p, ctypes.c_int, ctypes.c_int, ctypes.c_float, f32p]),
+    ("dm_dfs_config_init", ctypes.c_int, [ctypes.POINTER(DfsConfig)]),
+    ("dm_dfs_create", ctypes.c_int, [GP, ctypes.c_int, ctypes.c_int, ctypes.POINTER(DfsConfig), ctypes.POINTER(GP)]),
+    ("dm_dfs_kernel", ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]),
+    ("dm_dfs_run", ctypes.c_int, [GP, ctypes.c_void_p, u64p, ctypes.c_size_t, ctypes.POINTER(DfsStats)]),
+    ("dm_dfs_free", None, [GP]),
     ("dm_validate_counter", ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, ctypes.c_uint64,
                                            ctypes.POINTER(ctypes.c_int)]),
     ("dm_validate_pattern", ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, u64p, ctypes.c_size_t,
@@ -331,6 +357,53 @@ def mlp_adam(params, X, Y, steps, lr):
     _check(lib().dm_mlp_adam(_ptr(P, f32p), _ptr(X, f32p), _ptr(Y, f32p), len(Y), int(steps),
                              float(lr), _ptr(losses, f32p)))
     return P, losses
+
+
+def dfs_config(**kw):
+    """A dm_dfs_config with the library defaults, overridden by keyword."""
+    c = DfsConfig()
+    _check(lib().dm_dfs_config_init(ctypes.byref(c)))
+    for k, v in kw.items():
+        if not hasattr(c, k):
+            raise KeyError(k)
+        setattr(c, k, v)
+    return c
+
+
+def dfs_kernel(kind, vwave):
+    """Host stub of one of the library's step-API kernels (DM_DFS_*)."""
+    p = ctypes.c_void_p()
+    _check(lib().dm_dfs_kernel(int(kind), int(vwave), ctypes.byref(p)))
+    return p.value
+
+
+class Dfs:
+    """Per-worker DFS state of the step API (include/dumato_device.h) for one
+    (graph, k, clique|motif), run with host-driven load balancing."""
+
+    def __init__(self, graph, k, motif, cfg=None):
+        self.graph, self.k, self.motif = graph, int(k), bool(motif)
+        self.cfg = cfg if cfg is not None else dfs_config()
+        h = GP()
+        _check(lib().dm_dfs_create(graph._h, self.k, 1 if motif else 0, ctypes.byref(self.cfg), ctypes.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value and _lib is not None:
+            _lib.dm_dfs_free(h)
+            self._h = None
+
+    def run(self, kernel=None):
+        """Runs `kernel` (a host stub; default: the library's kernel of the
+        state's kind and width).  Returns (total, counts, stats)."""
+        if kernel is None:
+            kernel = dfs_kernel(DM_DFS_MOTIF if self.motif else DM_DFS_CLIQUE_PA, self.cfg.vwave or 32)
+        n = num_classes(self.k) if self.motif else 1
+        out = np.zeros(n, np.uint64)
+        st = DfsStats()
+        _check(lib().dm_dfs_run(self._h, ctypes.c_void_p(kernel), _ptr(out, u64p), n, ctypes.byref(st)))
+        return int(out.sum()), out, st.as_dict()
 
 
 class Query(ctypes.Structure):

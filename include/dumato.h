@@ -55,7 +55,8 @@ typedef struct dm_config {
     int32_t root_end;        /* ...(original vertex ids); root_end <= 0: all */
     int32_t clique_bitmap;   /* 1: k=4..6 cliques of roots with out-degree <= 256 count
                                 on a local adjacency bitmap; 0: list intersections */
-    int32_t reserved;
+    int32_t repartition;     /* 1: compute the static partition afresh (features,
+                                inference, binning) instead of reusing the cached one */
     double watchdog_s;       /* device-side time limit per launch (seconds) */
 } dm_config;
 
@@ -76,6 +77,8 @@ typedef struct dm_stats {
     double first_job_ms, static_done_ms, last_job_ms, exit_ms;
     uint64_t job_max_cycles;  /* cycles of the longest single job (a lower bound
                                  on the makespan that stealing cannot cut) */
+    double infer_ms;          /* learned: feature scaling and the MLP forward launch */
+    double bin_ms;            /* learned: splitting, ordering and binning the roots */
 } dm_stats;
 
 /* Fills *cfg with the defaults (plain scheduler, autotuned width/LDS). */
@@ -213,6 +216,94 @@ int dm_batch_autotune(dm_batch *b, const dm_query *queries, size_t nq,
                       double budget_ms, dm_config *best);
 int dm_batch_last_stats(const dm_batch *b, dm_stats *st);
 void dm_batch_free(dm_batch *b);
+
+/* ---- DFS-wide step API and host-driven load balancing ----
+ * The reference's programming model: a user kernel drives one DFS per
+ * (virtual) warp through the DuMatoGPU steps start / extend / filter /
+ * compact / aggregate / forward / backward (src/main/clique_PA.cu:9-32,
+ * src/main/motifs_DM_WCLB.cu:9-36, src/DM_WCV/clique_DM_WCV8.cu:8-34), and
+ * the host class DuMatoCPU launches it, polls idle warps, stops the kernel,
+ * moves work between warps and relaunches (src/main/DuMatoCPU.cu:316-562,
+ * src/main/EnumerationHelper.cpp:90-140, the loop of
+ * src/main/motifs_DM_WCLB.cu:85-120).  Here the steps are the header-only
+ * HIP API of include/dumato_device.h over per-worker state in device
+ * memory, and dm_dfs_run is the host loop.  (The fused kernels behind
+ * dm_clique_count / dm_motif_count are the fast path; this is the
+ * reference-shaped one.) */
+
+/* Per-worker DFS state in device memory, passed by value to the kernel.
+ * Vertex ids are the graph's degree ranks (ascending (degree, id)). */
+#define DM_DFS_MAXK 16
+typedef struct dm_dfs_data {
+    const int64_t *off;       /* symmetric CSR: off[n+1], adj, each list ascending */
+    const int32_t *adj;
+    const int64_t *ostart;    /* ostart[v]: first index of N(v) above v */
+    int32_t n, K, motif, workers;
+    int32_t vwave, jobs_per_worker;
+    int32_t cap_base;         /* list capacity unit: max degree (motif) or max out-degree */
+    int32_t ncls;             /* motif classes (block histogram slots) */
+    int64_t ext_per_worker;   /* int32 entries of a worker's level lists */
+    int32_t *ext;             /* level lists (workers x ext_per_worker) */
+    int8_t *src;              /* motif: first neighbourhood of each list entry */
+    int32_t *k;               /* [workers] current level; -1: take the next job */
+    int32_t *status;          /* [workers] 1 working, 2 idle (no job left) */
+    int32_t *id;              /* [workers x DM_DFS_MAXK] vertex at each level */
+    int32_t *next;            /* [workers x DM_DFS_MAXK] extensions left per level; -1: not generated */
+    int32_t *fixed;           /* [workers x DM_DFS_MAXK] full size of each level's list */
+    uint64_t *ind;            /* [workers x DM_DFS_MAXK] induction (quick bits) per level */
+    int32_t *jobs;            /* [workers x jobs_per_worker x DM_DFS_MAXK] job prefixes */
+    int32_t *job_level;       /* [workers x jobs_per_worker] */
+    uint64_t *job_ind;        /* [workers x jobs_per_worker] induction of the job's last level */
+    int32_t *job_valid;       /* [workers] jobs in the queue */
+    int32_t *job_cur;         /* [workers] next job to take */
+    const uint16_t *lut;      /* motif: quick code -> class */
+    unsigned long long *result; /* clique count, or the histogram (ncls slots) */
+    const int32_t *stop;      /* host sets 1: every worker leaves its loop at the next step */
+} dm_dfs_data;
+
+#define DM_DFS_CLIQUE_PA 0 /* clique_PA.cu: extend_clique / forward / backward_clique */
+#define DM_DFS_CLIQUE_WC 1 /* clique_DM_WCV8.cu: extend / filterClique / forward / backward */
+#define DM_DFS_MOTIF 2     /* motifs_DM_WCLB.cu: extend(0,k) / canonicalFilter2 / forward_induce */
+
+typedef struct dm_dfs_config {
+    int32_t vwave;           /* lanes per DFS worker: 8, 16, 32, 64 (0 = 32, the reference's warp) */
+    int32_t blocks;          /* 0 = as many as stay resident */
+    int32_t jobs_per_worker; /* job-queue slots per worker (reference donationsPerWarp); 0 = 16 */
+    int32_t threshold;       /* rebalance when more than threshold % of the workers are idle
+                                (reference argv threshold); >= 100: no load balancing */
+    int32_t poll_us;         /* host poll period of the idle count (reference: 100 ms); 0 = 1000 */
+    int32_t stop_every_us;   /* > 0: also stop, snapshot and rebalance this often */
+    int32_t max_rebalances;  /* 0 = unlimited */
+    int32_t reserved;
+    double watchdog_s;       /* host time limit of the whole run (0 = none) */
+} dm_dfs_config;
+
+typedef struct dm_dfs_stats {
+    double total_ms;         /* run time: launches, polls and rebalancing */
+    double lb_ms;            /* time spent stopped: snapshot, donation, copy back */
+    int32_t launches, rebalances, workers, vwave;
+    uint64_t donated_jobs;
+    uint64_t subgraphs;
+} dm_dfs_stats;
+
+typedef struct dm_dfs dm_dfs;
+int dm_dfs_config_init(dm_dfs_config *cfg);
+/* Device state for k-clique (motif = 0) or k-motif (motif = 1) enumeration
+ * of g with the step API (reference: the DuMatoCPU constructor,
+ * src/main/DuMatoCPU.cu:13-30, 204-276). */
+int dm_dfs_create(dm_graph *g, int k, int motif, const dm_dfs_config *cfg, dm_dfs **out);
+/* Host stub of a kernel of this library written against the step API
+ * (kind = DM_DFS_*) for the given width, for dm_dfs_run. */
+int dm_dfs_kernel(int kind, int vwave, const void **kernel);
+/* Runs `kernel` (a __global__ void(dm_dfs_data) built against
+ * include/dumato_device.h with the state's width) from the static partition
+ * (root v to worker v mod workers, by degree rank), with the host-driven load
+ * balancing of the config, until every job is done.  out: the clique count
+ * (out_len >= 1) or the histogram (out_len >= dm_motif_num_classes(k)).
+ * Replaces runKernel / gpuIsIdle / stopKernel / rebalance / waitKernel /
+ * outputAggregate* (src/main/DuMatoCPU.cu:316-562). */
+int dm_dfs_run(dm_dfs *s, const void *kernel, uint64_t *out, size_t out_len, dm_dfs_stats *st);
+void dm_dfs_free(dm_dfs *s);
 
 /* ---- validation against Peregrine output files ---- */
 /* Reads "<graph_path>.<k>.aggregateCounter" and sets *match = 1 when the sum
