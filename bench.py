@@ -2,39 +2,48 @@
 """Benchmark: subgraphs enumerated per second on one MI355X.
 
 Headline (value): k=4 motif histogram of the seeded Chung-Lu graph
-(n=18,000, 200,000 edges, gamma 2.3, wmax 500), plain scheduler.
+(n=18,000, 200,000 edges, gamma 2.3, wmax 500), plain scheduler, autotuned
+launch shape.  A step is one full counting call over the whole graph
+(schedule, kernel, result copy); the graph is resident on the GPU before
+timing starts.  value = subgraphs per second of wall time over the timed steps.
 
-Same JSON line, extra fields:
-  clique_rmat17   k=5 and k=6 clique counts of the seeded R-MAT scale-17
-                  graph, plain vs learned scheduler (the cost model is trained
-                  on per-root cycle counts of k=4 runs of the same graph; its
-                  Spearman/MAPE against measured per-root cycles at k=4..6 and
-                  the makespan reduction are reported);
-  learned_speedup plain / learned time per query, where time = enumeration
-                  kernel + host scheduling (inference, ordering, binning);
-  motif5_ba       (--full) k=5 motif histogram of the Barabasi-Albert graph
-                  with 200 planted cliques, plain vs learned.
+Same JSON line, extra blocks (--no-extra skips the first three):
+  er2000          k=3/k=4 cliques of the Erdos-Renyi n=2000 graph;
+  clique_rmat17   k=5 and k=6 cliques of the seeded R-MAT scale-17 graph;
+  motif5_ba       k=5 motifs of the Barabasi-Albert n=100000 graph with 200
+                  planted cliques.
+The last two compare the plain runtime at its default launch shape (full
+64-lane waves, default LDS stack, steal granularity 2), the plain runtime
+autotuned, and the learned scheduler autotuned.  The cost model is trained on
+per-root cycle counts of k=4 runs of the same graph.  Query time = kernel +
+host scheduling, and every timed learned query computes its partition afresh
+(feature standardisation, one MLP forward launch, splitting, binning), so
+inference and binning are inside it.  learned_speedup = plain (default shape)
+query time / learned (autotuned) query time.
+  batch256        (--no-batch skips it) 256 seeded graphs x {clique k=3..6,
+                  motif k=3..5} in one hipGraph-replayed launch.
 
-A step is one full counting call over the whole graph (schedule, kernel,
-result copy); the graph is resident on the GPU before timing starts and the
-launch parameters are autotuned once before timing.  value = subgraphs per
-second of wall time over the timed steps.  The roofline uses the enumeration
-kernel's own duration (HIP events on the library's stream, median over the
-steps) and its algorithmic bytes: 4 bytes per adjacency entry the DFS reads
-(each member list of every node, plus k-1 adjacency tests per motif leaf;
-for cliques the out-lists of the candidates of nodes with 1..k-2 vertices),
-summed on the device and checked against the CPU oracle in tests.  The
-traffic field is the measured HBM bytes per launch of the same kernel
-(rocprofv3 FETCH_SIZE and WRITE_SIZE, calibrated on a copy kernel), and
-traffic_GBs that traffic over the same kernel time.  The algorithmic fraction
-can exceed 1: the graph and the DFS lists are served from LDS and L2/MALL, and
-most leaf-level neighbour tests are answered by an LDS membership filter, so
-the kernel moves far fewer HBM bytes than the algorithmic figure counts.
+roofline: the resource that bounds the headline kernel.  Its shader counters
+(profiles/counters.json, from the rocprofv3 runs summarised in
+profiles/enum_kernel_motif4_chunglu18k.md) show VALU issue as the most used
+unit (latency-bound otherwise: waves wait about two thirds of their cycles),
+so achieved = VALU wave-instructions per launch (profile) / kernel time (HIP
+events on the library's stream, this run), against the MI355X issue peak of
+256 CUs x 4 SIMDs x 2.4 GHz / 4 cycles per wave64 instruction.  Beside it:
+traffic = HBM bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, calibrated
+on a copy of known size) and the algorithmic bytes of the enumeration
+(4 bytes per adjacency entry the DFS reads, plus k-1 adjacency tests per motif
+leaf; summed on the device and checked against the CPU oracle in tests), each
+as a rate and as a fraction of the 8 TB/s HBM peak.  The algorithmic bytes
+are mostly served from L2/MALL and LDS, and the leaf phase never performs the
+per-leaf adjacency tests the formula counts, so their rate is not an HBM
+figure.
 
-The CPU baseline is the oracle's port of the reference's DFS logic
-(clique_DM_DFS / motifs_DM_DFS: thread per root, linear-scan adjacency), with
-OpenMP over all host cores, timed on a seeded random sample of roots for a
-bounded time.
+cpu_baseline: the oracle's port of the reference's DFS logic (clique_DM_DFS /
+motifs_DM_DFS: thread per root, linear-scan adjacency tests) with OpenMP
+(schedule(dynamic,16), OMP_PROC_BIND=close) over the host cores, timed on the
+whole headline graph.  cpu_baselines lists every workload's CPU run (a time
+budget turns a run into a seeded random sample of roots, labelled so).
 """
 import argparse
 import json
@@ -45,9 +54,13 @@ import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
+# the CPU baseline's OpenMP threads stay close to their cores
+os.environ.setdefault("OMP_PROC_BIND", "close")
 
 HBM_PEAK_GBS = 8000.0
-PROFILE_TRAFFIC = os.path.join(ROOT, "profiles", "traffic.json")
+# VALU issue peak: 256 CUs x 4 SIMDs, one wave64 instruction per 4 cycles, 2.4 GHz
+VALU_PEAK_GINST = 256 * 4 * 2.4 / 4
+PROFILE_COUNTERS = os.path.join(ROOT, "profiles", "counters.json")
 
 
 def progress(msg):
@@ -58,38 +71,63 @@ def progress(msg):
 T0 = time.perf_counter()
 
 
-def cpu_baseline(graph, mode, k, budget_s, seed=7):
-    """Oracle port timed on random roots until budget_s is used."""
+def host_cpu():
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return model
+
+
+def cpu_run(graph, mode, k, budget_s, faithful=True, oriented=False, seed=7, label="", chunk_cap=8192,
+            chunk_s=8.0):
+    """Oracle port over the graph's roots (seeded random order, in chunks) until
+    all are done or budget_s is used.  Chunks at most double and stay below
+    chunk_s seconds of estimated work: large enough to keep every thread busy
+    (dynamic scheduling inside a chunk), small enough that a chunk of
+    unexpectedly heavy roots cannot run long unreported.  oriented: the FAST
+    variant's input, the degree-oriented relabelled CSR (cliques)."""
     import numpy as np
 
     from oracle import pyoracle as O
     n, m, _ = graph.info()
     off, adj = graph.csr()
     ref = O.CSR(n, off, adj)
+    if oriented:
+        _, ref = O.relabel(ref, oriented=True)
     order = np.random.default_rng(seed).permutation(n).astype(np.int32)
     threads = O.num_threads()
-    done = 0
-    count = 0
-    chunk = 64
+    done = count = 0
+    chunk = 16
     t0 = time.perf_counter()
     while done < n and time.perf_counter() - t0 < budget_s:
         roots = order[done:done + chunk]
         if mode == "motif":
-            c, _, _ = O.motif_count_roots(ref, k, roots, nthreads=threads, faithful=True)
+            c, _, _ = O.motif_count_roots(ref, k, roots, nthreads=threads, faithful=faithful)
         else:
-            c, _ = O.clique_count_roots(ref, k, roots, nthreads=threads, faithful=True)
+            c, _ = O.clique_count_roots(ref, k, roots, nthreads=threads, faithful=faithful)
         count += c
         done += len(roots)
         el = time.perf_counter() - t0
         per_root = el / done
-        left = budget_s - el
-        chunk = int(max(16, min(4096, 0.9 * left / per_root if per_root > 0 else 4096)))
-        progress("cpu baseline: %d roots, %.1f s" % (done, el))
+        est = chunk_s / per_root if per_root > 0 else chunk_cap
+        chunk = int(max(16, min(chunk_cap, 2 * chunk, est)))
+        progress("cpu %s %s k=%d: %d/%d roots, %.1f s" % (label, mode, k, done, n, el))
     dt = time.perf_counter() - t0
-    return {"value": count / dt if dt > 0 else 0.0, "unit": "subgraphs/s", "cores": threads,
-            "kind": "port",
-            "sample": "%d of %d roots (seeded random order), %d subgraphs in %.1f s, "
-                      "faithful linear-scan DFS of the reference (%s k=%d)" % (done, n, count, dt, mode, k)}
+    full = done >= n
+    variant = "faithful linear-scan adjacency" if faithful else "binary-search adjacency (FAST)"
+    if oriented:
+        variant += " on the degree-oriented relabelled CSR"
+    return {"workload": label, "query": "%s k=%d" % (mode, k), "value": count / dt if dt > 0 else 0.0,
+            "unit": "subgraphs/s", "cores": threads, "kind": "port", "variant": variant,
+            "subgraphs": count, "seconds": dt, "roots": done, "of_roots": n, "full": full,
+            "sample": ("all %d roots" % n) if full else
+                      ("%d of %d roots (seeded random order; time budget %.0f s)" % (done, n, budget_s))}
 
 
 def timed(fn, graph, warmup, steps):
@@ -107,18 +145,17 @@ def timed(fn, graph, warmup, steps):
     return out, time.perf_counter() - t0, kms, sms
 
 
-def query_block(graph, mode, k, sched, warmup, steps):
-    import dumato_amd as dm
-    cfg = dm.config(sched=sched)
+def query_block(graph, mode, k, cfg, warmup, steps):
     fn = (lambda: graph.motif_count(k, cfg)[0]) if mode == "motif" else (lambda: graph.clique_count(k, cfg))
     count, wall, kms, sms = timed(fn, graph, warmup, steps)
     st = graph.last_stats()
     km, sm = statistics.median(kms), statistics.median(sms)
     return {"subgraphs": count, "kernel_ms": km, "sched_ms": sm, "query_ms": km + sm,
             "subgraphs_per_s": count / ((km + sm) / 1e3), "wall_ms_per_step": 1e3 * wall / steps,
-            "vwave": st["vwave"], "workers": st["workers"], "donations": st["donations"],
+            "vwave": st["vwave"], "workers": st["workers"], "lds_bytes": st["lds_bytes"],
+            "donations": st["donations"],
             "balance": st["busy_sum_cycles"] / max(1, st["workers"] * st["busy_max_cycles"]),
-            "busy_max_mcycles": st["busy_max_cycles"] / 1e6,
+            "busy_max_mcycles": st["busy_max_cycles"] / 1e6, "job_max_mcycles": st["job_max_cycles"] / 1e6,
             "alg_GBs": st["alg_bytes"] / (km / 1e3) / 1e9}
 
 
@@ -153,31 +190,46 @@ def model_quality(graph, mode, k):
             "mape": float(np.mean(np.abs(est - cost[has]) / cost[has]))}
 
 
-def plain_vs_learned(graph, mode, k_list, k_train, warmup, steps, budget_ms=300.0):
+def plain_vs_learned(graph, mode, k_list, k_train, warmup, steps, budget_ms=300.0, quality=True):
+    """Plain at the default shape, plain autotuned, learned autotuned."""
     import dumato_amd as dm
     out = {}
+    motif = mode == "motif"
+    default_cfgs = {}
+    for k in k_list:
+        # the default shape, fixed before any tuning is cached in the graph
+        default_cfgs[k] = dm.config(vwave=64, steal_min=2, repartition=1)
+        out["k%d" % k] = {"plain_default": query_block(graph, mode, k, default_cfgs[k], warmup, steps)}
+        progress("%s k=%d plain default shape: kernel %.2f ms" % (mode, k, out["k%d" % k]["plain_default"]["kernel_ms"]))
     t = time.perf_counter()
-    loss = graph.sched_train(mode == "motif", k_train, epochs=30, seed=1)
+    loss = graph.sched_train(motif, k_train, epochs=30, seed=1)
     out["train"] = {"k_train": k_train, "loss": loss, "seconds": time.perf_counter() - t}
     progress("%s: cost model trained on k=%d (loss %.3f)" % (mode, k_train, loss))
-    out["model_vs_measured"] = [model_quality(graph, mode, k) for k in (k_train,) + tuple(k_list)]
-    progress("%s: model quality measured" % mode)
+    if quality:
+        out["model_vs_measured"] = [model_quality(graph, mode, k) for k in (k_train,) + tuple(k_list)]
+        progress("%s: model quality measured" % mode)
     for k in k_list:
-        graph.autotune(mode == "motif", k, budget_ms)
+        rec = out["k%d" % k]
+        graph.autotune(motif, k, budget_ms)
         progress("%s k=%d: autotuned" % (mode, k))
-        p = query_block(graph, mode, k, dm.DM_SCHED_PLAIN, warmup, steps)
-        progress("%s k=%d plain: kernel %.2f ms" % (mode, k, p["kernel_ms"]))
-        lq = query_block(graph, mode, k, dm.DM_SCHED_LEARNED, warmup, steps)
-        progress("%s k=%d learned: kernel %.2f ms" % (mode, k, lq["kernel_ms"]))
-        assert p["subgraphs"] == lq["subgraphs"], "learned and plain counts differ"
-        out["k%d" % k] = {"plain": p, "learned": lq,
-                          "learned_speedup": p["query_ms"] / lq["query_ms"],
-                          "learned_kernel_speedup": p["kernel_ms"] / lq["kernel_ms"],
-                          "makespan_reduction": 1.0 - lq["busy_max_mcycles"] / p["busy_max_mcycles"]}
+        rec["plain_tuned"] = query_block(graph, mode, k, dm.config(repartition=1), warmup, steps)
+        progress("%s k=%d plain tuned: kernel %.2f ms" % (mode, k, rec["plain_tuned"]["kernel_ms"]))
+        rec["learned_tuned"] = query_block(graph, mode, k, dm.config(sched=dm.DM_SCHED_LEARNED, repartition=1),
+                                           warmup, steps)
+        progress("%s k=%d learned tuned: kernel %.2f ms, sched %.2f ms" %
+                 (mode, k, rec["learned_tuned"]["kernel_ms"], rec["learned_tuned"]["sched_ms"]))
+        counts = {rec[x]["subgraphs"] for x in ("plain_default", "plain_tuned", "learned_tuned")}
+        assert len(counts) == 1, "learned and plain counts differ"
+        pd, pt, lt = rec["plain_default"], rec["plain_tuned"], rec["learned_tuned"]
+        rec["learned_speedup"] = pd["query_ms"] / lt["query_ms"]
+        rec["learned_vs_plain_tuned"] = pt["query_ms"] / lt["query_ms"]
+        rec["makespan_reduction"] = 1.0 - lt["busy_max_mcycles"] / pd["busy_max_mcycles"]
+        # what scheduling could gain at most: the plain runtime's idle share
+        rec["plain_balance_ceiling"] = 1.0 / max(pd["balance"], 1e-9)
     return out
 
 
-def batch_block(warmup, steps, check_graphs=8):
+def batch_block(warmup, steps):
     """256 seeded graphs x {clique k=3..6, motif k=3..5} in one hipGraph-replayed launch."""
     import numpy as np
 
@@ -204,12 +256,6 @@ def batch_block(warmup, steps, check_graphs=8):
     wall = time.perf_counter() - t0
     progress("batch: %d replays timed" % steps)
     total = sum(int(np.asarray(r).sum()) for r in res)
-    # spot-check: the first graphs' results against the single-graph path
-    ok = True
-    for gi in range(check_graphs):
-        for (mode, k), r in zip(W.BATCH_QUERIES, res[gi * len(W.BATCH_QUERIES):(gi + 1) * len(W.BATCH_QUERIES)]):
-            one = graphs[gi].motif_count(k)[1] if mode == "motif" else graphs[gi].clique_count(k)
-            ok = ok and np.array_equal(np.asarray(one), np.asarray(r))
     km = statistics.median(kms)
     lat = np.median(np.stack(lats), axis=0)
     st = b.last_stats()
@@ -217,8 +263,26 @@ def batch_block(warmup, steps, check_graphs=8):
             "kernel_ms": km, "subgraphs_per_s": total / (km / 1e3), "wall_ms_per_step": 1e3 * wall / steps,
             "latency_ms_p50": float(np.percentile(lat, 50)), "latency_ms_p99": float(np.percentile(lat, 99)),
             "vwave": st["vwave"], "workers": st["workers"], "steal_min": best.steal_min,
-            "donations": st["donations"], "graph_build_s": gen_s,
-            "spot_check_vs_single_graph_path": ok, "hipgraph": True}
+            "donations": st["donations"], "graph_build_s": gen_s, "hipgraph": True,
+            "parity": "every query pinned against the oracle by tests/test_gpu_zbatch256.py"}
+
+
+def roofline(st, kern_ms, total):
+    rec = {}
+    if os.path.exists(PROFILE_COUNTERS):
+        rec = json.load(open(PROFILE_COUNTERS)).get("motif4_chunglu18k", {})
+    valu = rec.get("valu_insts")
+    traffic = rec.get("hbm_bytes")
+    t = kern_ms / 1e3
+    achieved = valu / t / 1e9 if valu else None
+    alg = st["alg_bytes"]
+    return {"bound": "valu", "achieved": achieved, "peak": VALU_PEAK_GINST, "unit": "Ginst/s",
+            "frac": achieved / VALU_PEAK_GINST if achieved else None, "traffic": traffic,
+            "valu_insts_per_launch": valu, "counters": "profiles/enum_kernel_motif4_chunglu18k.md",
+            "hbm_GBs": traffic / t / 1e9 if traffic else None,
+            "hbm_frac": traffic / t / 1e9 / HBM_PEAK_GBS if traffic else None,
+            "alg_bytes": alg, "alg_GBs": alg / t / 1e9, "alg_frac_of_hbm_peak": alg / t / 1e9 / HBM_PEAK_GBS,
+            "bytes_per_subgraph": alg / max(1, total)}
 
 
 def main():
@@ -226,10 +290,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=7)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--cpu-budget", type=float, default=150.0,
+                    help="seconds per CPU baseline run before it becomes a sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
-    ap.add_argument("--full", action="store_true", help="also the k=5 motif run on the BA graph (~1 min)")
     ap.add_argument("--no-batch", action="store_true", help="skip the 256-graph batched launch")
     ap.add_argument("--no-autotune", action="store_true",
                     help="headline with the default launch shape (no tuning dispatches; for profiling)")
@@ -254,15 +318,10 @@ def main():
         return g.motif_count(K, cfg)[0]
 
     total, elapsed, kms, _ = timed(step, g, args.warmup, args.steps)
-    progress("headline: %d steps timed, kernel %.2f ms" % (args.steps, statistics.median(kms)))
+    kern_ms = statistics.median(kms)
+    progress("headline: %d steps timed, kernel %.2f ms" % (args.steps, kern_ms))
     st = g.last_stats()
     value = args.steps * total / elapsed
-    kern_ms = statistics.median(kms)
-    achieved = st["alg_bytes"] / (kern_ms / 1e3) / 1e9
-    traffic = None
-    if os.path.exists(PROFILE_TRAFFIC):
-        rec = json.load(open(PROFILE_TRAFFIC)).get("motif4_chunglu18k")
-        traffic = rec["bytes"] if rec else None
     line = {
         "metric": "subgraphs enumerated/sec (k=4 motif on Chung-Lu 18K; k=5 clique and learned speedup below)",
         "value": value,
@@ -284,25 +343,34 @@ def main():
                    "subgraphs_per_kernel_s": total / (kern_ms / 1e3),
                    "balance": st["busy_sum_cycles"] / max(1, st["workers"] * st["busy_max_cycles"]),
                    "donations": st["donations"]},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "alg_bytes": st["alg_bytes"],
-                     "bytes_per_subgraph": st["alg_bytes"] / max(1, total),
-                     "traffic_GBs": (traffic / (kern_ms * 1e6)) if traffic else None},
+        "roofline": roofline(st, kern_ms, total),
     }
     if not args.no_extra:
+        ge = W.er_2000()
+        line["er2000"] = {"k%d" % k: query_block(ge, "clique", k, dm.config(), 1, 3) for k in (3, 4)}
+        progress("er2000: cliques k=3/4")
         gr = W.rmat_s17()
         line["clique_rmat17"] = plain_vs_learned(gr, "clique", (5, 6), 4, args.warmup, args.steps)
+        gb = W.ba_100k_planted()
+        line["motif5_ba"] = plain_vs_learned(gb, "motif", (5,), 4, 0, 1, budget_ms=2000.0, quality=False)
         line["learned_speedup"] = {"clique_k5_rmat17": line["clique_rmat17"]["k5"]["learned_speedup"],
-                                   "clique_k6_rmat17": line["clique_rmat17"]["k6"]["learned_speedup"]}
+                                   "clique_k6_rmat17": line["clique_rmat17"]["k6"]["learned_speedup"],
+                                   "motif_k5_ba": line["motif5_ba"]["k5"]["learned_speedup"]}
     if not args.no_batch:
         line["batch256"] = batch_block(min(args.warmup, 1), max(1, args.steps // 3))
-    if args.full:
-        gb = W.ba_100k_planted()
-        line["motif5_ba"] = plain_vs_learned(gb, "motif", (5,), 4, 0, 1, budget_ms=2000.0)
-        line.setdefault("learned_speedup", {})["motif_k5_ba"] = line["motif5_ba"]["k5"]["learned_speedup"]
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(g, "motif", K, args.cpu_budget)
+        runs = [cpu_run(g, "motif", K, args.cpu_budget, label="chunglu_18k")]
+        line["cpu_baseline"] = dict(runs[0])
+        if not args.no_extra:
+            runs.append(cpu_run(ge, "clique", 4, args.cpu_budget, label="er_2000"))
+            # the faithful port needs hours here (hub neighbourhoods scanned
+            # linearly): a seeded sample, beside the FAST variant on all roots
+            runs.append(cpu_run(gr, "clique", 5, 20.0, label="rmat_s17", chunk_cap=64, chunk_s=1.0))
+            runs.append(cpu_run(gr, "clique", 5, args.cpu_budget, faithful=False, oriented=True, label="rmat_s17"))
+        host = {"cpu_model": host_cpu(), "nproc": os.cpu_count(), "omp_threads": runs[0]["cores"],
+                "OMP_PROC_BIND": os.environ.get("OMP_PROC_BIND")}
+        line["cpu_baseline"].update(host)
+        line["cpu_baselines"] = runs
     print(json.dumps(line), flush=True)
 
 

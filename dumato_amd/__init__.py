@@ -73,6 +73,10 @@ class DfsStats(ctypes.Structure):
 DM_DFS_CLIQUE_PA = 0
 DM_DFS_CLIQUE_WC = 1
 DM_DFS_MOTIF = 2
+DM_DFS_NEAR_CLIQUE = 3
+DM_PATTERN_CLIQUE = 0
+DM_PATTERN_MOTIF = 1
+DM_PATTERN_NEAR_CLIQUE = 2
 
 _lib = None
 
@@ -379,13 +383,17 @@ def dfs_kernel(kind, vwave):
 
 class Dfs:
     """Per-worker DFS state of the step API (include/dumato_device.h) for one
-    (graph, k, clique|motif), run with host-driven load balancing."""
+    (graph, k, pattern), run with host-driven load balancing.  pattern:
+    DM_PATTERN_CLIQUE / DM_PATTERN_MOTIF / DM_PATTERN_NEAR_CLIQUE (False /
+    True stand for clique / motif)."""
 
-    def __init__(self, graph, k, motif, cfg=None):
-        self.graph, self.k, self.motif = graph, int(k), bool(motif)
+    def __init__(self, graph, k, pattern, cfg=None):
+        pattern = int(pattern)
+        self.graph, self.k, self.pattern = graph, int(k), pattern
+        self.motif = pattern == DM_PATTERN_MOTIF
         self.cfg = cfg if cfg is not None else dfs_config()
         h = GP()
-        _check(lib().dm_dfs_create(graph._h, self.k, 1 if motif else 0, ctypes.byref(self.cfg), ctypes.byref(h)))
+        _check(lib().dm_dfs_create(graph._h, self.k, pattern, ctypes.byref(self.cfg), ctypes.byref(h)))
         self._h = h
 
     def __del__(self):
@@ -398,7 +406,9 @@ class Dfs:
         """Runs `kernel` (a host stub; default: the library's kernel of the
         state's kind and width).  Returns (total, counts, stats)."""
         if kernel is None:
-            kernel = dfs_kernel(DM_DFS_MOTIF if self.motif else DM_DFS_CLIQUE_PA, self.cfg.vwave or 32)
+            kind = {DM_PATTERN_CLIQUE: DM_DFS_CLIQUE_PA, DM_PATTERN_MOTIF: DM_DFS_MOTIF,
+                    DM_PATTERN_NEAR_CLIQUE: DM_DFS_NEAR_CLIQUE}[self.pattern]
+            kernel = dfs_kernel(kind, self.cfg.vwave or 32)
         n = num_classes(self.k) if self.motif else 1
         out = np.zeros(n, np.uint64)
         st = DfsStats()

@@ -88,9 +88,14 @@ static bool host_adjacent(const HostCSR &c, int32_t u, int32_t x) {
     return p != e && *p == x;
 }
 
-static int dfs_create(dm_graph *g, int K, bool motif, const dm_dfs_config &cfg, dm_dfs **out) {
+static int dfs_create(dm_graph *g, int K, int pattern, const dm_dfs_config &cfg, dm_dfs **out) {
+    const bool motif = pattern == DM_PATTERN_MOTIF, nearq = pattern == DM_PATTERN_NEAR_CLIQUE;
+    if (pattern < DM_PATTERN_CLIQUE || pattern > DM_PATTERN_NEAR_CLIQUE)
+        return set_error(DM_ERR_ARG, "unknown pattern %d", pattern);
     if (motif) {
         if (K < 3 || K > kMaxMotifK) return set_error(DM_ERR_UNSUPPORTED, "motif size k=%d unsupported (3..%d)", K, kMaxMotifK);
+    } else if (nearq) {
+        if (K < 4 || K > 13) return set_error(DM_ERR_ARG, "near-clique query size k=%d unsupported (4..13)", K);
     } else if (K < 2 || K > DM_DFS_MAXK - 1) {
         return set_error(DM_ERR_ARG, "clique size k=%d unsupported (2..%d)", K, DM_DFS_MAXK - 1);
     }
@@ -116,7 +121,10 @@ static int dfs_create(dm_graph *g, int K, bool motif, const dm_dfs_config &cfg, 
     const int32_t n = g->rcsr.n;
     s->J = std::max<int>(cfg.jobs_per_worker ? cfg.jobs_per_worker : 16, (n + s->workers - 1) / s->workers);
     s->ncls = motif ? tab->ncls : 0;
-    const int32_t cap = motif ? std::max(g->rcsr.maxdeg, 1) : std::max(g->max_out, 1);
+    // list capacity per level: a union of up to l+1 neighbourhoods (motif), a
+    // neighbourhood (near-clique: the common neighbours of the clique), or
+    // an out-neighbourhood (clique)
+    const int32_t cap = motif || nearq ? std::max(g->rcsr.maxdeg, 1) : std::max(g->max_out, 1);
     const int64_t per = motif ? (int64_t)cap * K * (K - 1) / 2 : (int64_t)cap * (K - 1);
     const size_t W = (size_t)s->workers, WM = W * DM_DFS_MAXK, WJ = W * (size_t)s->J;
     dm_dfs_data &d = s->d;
@@ -416,13 +424,13 @@ int dm_dfs_config_init(dm_dfs_config *cfg) {
     return DM_OK;
 }
 
-int dm_dfs_create(dm_graph *g, int k, int motif, const dm_dfs_config *cfg, dm_dfs **out) {
+int dm_dfs_create(dm_graph *g, int k, int pattern, const dm_dfs_config *cfg, dm_dfs **out) {
     if (!g || !out) return set_error(DM_ERR_ARG, "graph or out is NULL");
     *out = nullptr;
     dm_dfs_config c;
     if (cfg) c = *cfg;
     else dm_dfs_config_init(&c);
-    return dfs_create(g, k, motif != 0, c, out);
+    return dfs_create(g, k, pattern, c, out);
 }
 
 int dm_dfs_run(dm_dfs *s, const void *kernel, uint64_t *out, size_t out_len, dm_dfs_stats *st) {

@@ -3,6 +3,7 @@
 //   dfs_clique_pa   src/main/clique_PA.cu:9-32
 //   dfs_clique_wc   src/DM_WCV/clique_DM_WCV8.cu:8-34
 //   dfs_motifs      src/main/motifs_DM_WCLB.cu:9-36
+//   dfs_q_pa        src/main/q4_PA.cu:9-32 .. q13_PA.cu (near-clique query)
 // They run under dm_dfs_run (host-driven load balancing).  The fused kernels
 // of enum_kernel.hip compute the same counts faster; these exist so that the
 // reference's programming model, and its stop/snapshot/donate balancing,
@@ -78,12 +79,35 @@ __global__ __launch_bounds__(256) void dfs_motifs(dm_dfs_data d) {
     DM.end();
 }
 
+// q4_PA .. q13_PA (src/main/q4_PA.cu:9-32): the near-clique query with the
+// state's K
+template <int W>
+__global__ __launch_bounds__(256) void dfs_q_pa(dm_dfs_data d) {
+    dm_device::Worker<W> DM;
+    DM.start_q(d);
+    while (DM.active() && DM.balanced()) {
+        if (DM.getCurrentNumberOfExtensions() == -1) DM.extend_q();
+        if (DM.getCurrentNumberOfExtensions() != 0) {
+            if (DM.last_level()) {
+                DM.aggregate_counter();
+                DM.backward_q();
+            } else {
+                DM.forward();
+            }
+        } else {
+            DM.backward_q();
+        }
+    }
+    DM.end();
+}
+
 template <int W>
 static const void *dfs_kernel_w(int kind) {
     switch (kind) {
         case DM_DFS_CLIQUE_PA: return (const void *)dfs_clique_pa<W>;
         case DM_DFS_CLIQUE_WC: return (const void *)dfs_clique_wc<W>;
         case DM_DFS_MOTIF: return (const void *)dfs_motifs<W>;
+        case DM_DFS_NEAR_CLIQUE: return (const void *)dfs_q_pa<W>;
         default: return nullptr;
     }
 }

@@ -178,6 +178,8 @@ struct Worker {
     unsigned long long *shist;       // class counters the leaf phase adds to
     const uint16_t *lut;
     int qid;
+    int32_t qword;  // the current job's query word (query index | donation threshold)
+    int job_sm;     // the current job's donation threshold
     bool qmotif;
     uint32_t epoch;
     char *lds_lo, *lds_top;
@@ -219,6 +221,8 @@ struct Worker {
         g_lo = g_top = p.arena + (int64_t)wid * p.arena_per_worker;
         K = 0;
         qid = -1;
+        qword = 0;
+        job_sm = p.steal_min;
         qmotif = MODE == 1;
         lut = nullptr;
         epoch = (uint32_t)*p.epoch;
@@ -329,7 +333,7 @@ struct Worker {
             if (j == 0) v = lvl;
             else if (j == 1) v = split_at(lvl, cur, end, first + r, np + 1);
             else if (j == 2) v = split_at(lvl, cur, end, first + r + 1, np + 1);
-            else if (j == 3) v = qid;
+            else if (j == 3) v = qword;
             else v = (j - 4 <= lvl) ? ws.lv()[j - 4].vtx : -1;
             __hip_atomic_store(p.q[slot + r].w + j, tag | (uint32_t)v, __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_AGENT);
@@ -345,7 +349,7 @@ struct Worker {
         if (reserve_slots(1, slot) == 0) return false;
         const uint64_t tag = (uint64_t)epoch << 32;
         for (int j = g.gl; j < kJobWords; j += W) {
-            const int32_t v = j == 1 ? -1 : j == 2 ? hi : j == 3 ? qid : j == 4 ? lo : j == 0 ? 0 : -1;
+            const int32_t v = j == 1 ? -1 : j == 2 ? hi : j == 3 ? qword : j == 4 ? lo : j == 0 ? 0 : -1;
             __hip_atomic_store(p.q[slot].w + j, tag | (uint32_t)v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
         if (g.gl == 0) ws.st()[kStDonated]++;
@@ -420,10 +424,10 @@ struct Worker {
             return;
         }
         // the last level's children are cheap: split it only in bigger pieces
-        const int leaf_min = 4 * p.steal_min;
+        const int leaf_min = 4 * job_sm;
         for (int l = job_lvl; l < top; l++) {
             const int cur = ws.lv()[l].cur, end = ws.lv()[l].end;
-            const int minc = (min_first > 0 && l == job_lvl) ? min_first : l == K - 3 ? leaf_min : p.steal_min;
+            const int minc = (min_first > 0 && l == job_lvl) ? min_first : l == K - 3 ? leaf_min : job_sm;
             if (end - cur >= minc) {
                 int P = (end - cur) / minc - 1;
                 if (P < 1) P = 1;
@@ -924,6 +928,38 @@ struct Worker {
         uint64_t *rows = (uint64_t *)alloc((int64_t)d * NW * 8);
         int32_t *dp = (int32_t *)alloc((int64_t)d * 4);
         int32_t *pre = (int32_t *)alloc((int64_t)(d + 1) * 4);
+        if (abort) return;
+        // a root whose bitmap the launch prebuilt (a heavy root cut into
+        // many pieces): copy its rows instead of building them
+        int slot = -1;
+        if (p.npre > 0) {
+            int a = 0, b = p.npre;
+            while (a < b) {
+                const int mid = (a + b) >> 1;
+                if (p.pre_roots[mid] < v) a = mid + 1;
+                else b = mid;
+            }
+            if (a < p.npre && p.pre_roots[a] == v) slot = a;
+        }
+        if (slot >= 0) {
+            const uint64_t *gr = (const uint64_t *)(p.pre + p.pre_offs[slot]);
+            const int32_t *gd = (const int32_t *)(gr + (int64_t)d * NW);
+            for (int t = lo * NW + g.gl; t < d * NW; t += W) rows[t] = gr[t];
+            for (int i = g.gl; i < d; i += W) dp[i] = gd[i];
+            wsync();
+        } else {
+            build_rows<NW>(v, lo, d, L, rows, dp, pre);
+            if (abort) return;
+        }
+        if (first_piece && g.gl == 0) bytes += 4ull * (unsigned long long)d;
+        pair_phase<NW>(R, rows, dp, pre, lo, hi);
+    }
+
+    // Rows lo..d-1 of the local bitmap of root v (L = N+(v), d = |L|): row i
+    // bit j set when L[j] is in N+(L[i]); dp[i] = out-degree of L[i].
+    template <int NW>
+    DEV void build_rows(int32_t v, int lo, int d, const int32_t *L, uint64_t *rows, int32_t *dp, int32_t *pre) {
+        (void)v;
         // build scratch, released after the build: a hash of N+(v) (id ->
         // position, load factor <= 1/2) and each child's list start
         char *const build_lds = lds_top;
@@ -999,9 +1035,12 @@ struct Worker {
         wsync();
         lds_top = build_lds;
         g_top = build_glb;
-        if (first_piece && g.gl == 0) bytes += 4ull * (unsigned long long)d;
-        // first-level children [lo, hi) in batches of W; each batch is
-        // flattened into its (i, j) pairs, which the lanes stride over
+    }
+
+    // first-level children [lo, hi) in batches of W; each batch is
+    // flattened into its (i, j) pairs, which the lanes stride over
+    template <int NW>
+    DEV void pair_phase(Lvl &R, const uint64_t *rows, const int32_t *dp, int32_t *pre, int lo, int hi) {
         R.cur = lo;
         R.end = hi;
         unsigned long long by = 0;
@@ -1406,12 +1445,17 @@ __global__ __launch_bounds__(kThreads, (min_waves<W, MODE>())) void enum_kernel(
                 continue;
             }
             backoff = 1;
-            bool ok = jh.q >= 0 && jh.q < p.nq;
-            if (ok && jh.q != wk.qid) {
+            const int jq = job_query(jh.q);
+            bool ok = jh.q >= 0 && jq < p.nq;
+            // the job's own donation threshold travels with its query word
+            // to every piece it gives away
+            wk.qword = jh.q;
+            wk.job_sm = job_steal_min(jh.q) >= 2 ? job_steal_min(jh.q) : p.steal_min;
+            if (ok && jq != wk.qid) {
                 if (!single) flush_query<W>(p, g, wk.qid, ws.st()[kStQCount], whist, ws.st()[kStLast]);
                 if (g.gl == 0) ws.st()[kStQCount] = 0;
-                const QueryDesc &Q = p.qd[jh.q];
-                wk.qid = jh.q;
+                const QueryDesc &Q = p.qd[jq];
+                wk.qid = jq;
                 wk.K = Q.K;
                 wk.qmotif = Q.motif != 0;
                 wk.lut = Q.lut;
@@ -1509,6 +1553,73 @@ __global__ __launch_bounds__(kThreads, (min_waves<W, MODE>())) void enum_kernel(
         atomicMax(&p.sc->job_max, f[9]);
         atomicMax(&p.sc->t_exit, (unsigned long long)wall_clock64());
     }
+}
+
+// Local bitmaps of the launch's prebuilt roots, one block per root, in the
+// layout clique_bitmap_root reads: rows (d x NW uint64, row i bit j set when
+// L[j] is in N+(L[i]), L = N+(v)) then the out-degrees of L (d int32).  The
+// host lists only roots with out-degree <= 256.
+__global__ __launch_bounds__(256) void prebuild_kernel(EnumParams p) {
+    __shared__ int32_t hkey[512];
+    __shared__ uint8_t hval[512];
+    __shared__ int64_t rst[256];
+    __shared__ int32_t spre[257];
+    const int b = blockIdx.x, tid = threadIdx.x;
+    if (b >= p.npre) return;
+    const int32_t v = p.pre_roots[b];
+    const int d = (int)(p.off[v + 1] - p.ostart[v]);
+    if (d > 256) return;
+    const int NW = d <= 64 ? 1 : d <= 128 ? 2 : 4;
+    const int HB = NW == 1 ? 7 : NW == 2 ? 8 : 9;
+    const int HS = 1 << HB;
+    const int32_t *L = p.adj + p.ostart[v];
+    uint64_t *rows = (uint64_t *)const_cast<char *>(p.pre + p.pre_offs[b]);
+    int32_t *gdp = (int32_t *)(rows + (int64_t)d * NW);
+    for (int t = tid; t < HS; t += blockDim.x) hkey[t] = -1;
+    for (int i = tid; i < d; i += blockDim.x) {
+        const int32_t u = L[i];
+        const int64_t os = p.ostart[u];
+        rst[i] = os;
+        gdp[i] = (int)(p.off[u + 1] - os);
+    }
+    for (int t = tid; t < d * NW; t += blockDim.x) rows[t] = 0;
+    __syncthreads();
+    for (int j = tid; j < d; j += blockDim.x) {
+        const int32_t key = L[j];
+        uint32_t h = ((uint32_t)key * 2654435761u) >> (32 - HB);
+        while (atomicCAS(&hkey[h], -1, key) != -1) h = (h + 1) & (HS - 1);
+        hval[h] = (uint8_t)j;
+    }
+    if (tid == 0) {
+        spre[0] = 0;
+        for (int i = 0; i < d; i++) spre[i + 1] = spre[i] + (int)(p.off[L[i] + 1] - rst[i]);
+    }
+    __syncthreads();
+    const int T = spre[d];
+    for (int t = tid; t < T; t += blockDim.x) {
+        int a = 0, e = d - 1;  // the row of element t: spre[a] <= t < spre[a + 1]
+        while (a < e) {
+            const int mid = (a + e + 1) >> 1;
+            if (spre[mid] <= t) a = mid;
+            else e = mid - 1;
+        }
+        const int32_t x = p.adj[rst[a] + (t - spre[a])];
+        uint32_t h = ((uint32_t)x * 2654435761u) >> (32 - HB);
+        int k;
+        while ((k = hkey[h]) != x && k != -1) h = (h + 1) & (HS - 1);
+        if (k == x) {
+            const int j = hval[h];
+            atomicOr(&rows[(int64_t)a * NW + (j >> 6)], 1ull << (j & 63));
+        }
+    }
+}
+
+int prebuild_launch(const EnumParams &p, void *stream) {
+    if (p.npre <= 0) return DM_OK;
+    hipLaunchKernelGGL(prebuild_kernel, dim3(p.npre), dim3(256), 0, (hipStream_t)stream, p);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return set_error(DM_ERR_HIP, "prebuild launch: %s", hipGetErrorString(e));
+    return DM_OK;
 }
 
 __global__ void fill_u32_kernel(uint32_t *ptr, uint32_t value, size_t count) {

@@ -158,7 +158,22 @@ struct EnumParams {
     int32_t steal, steal_min, bitmap, pad4;
     uint64_t poll_ticks;              // wall-clock ticks between a worker's donation checks
     uint64_t watchdog_ticks;          // wall-clock ticks
+    // local bitmaps prebuilt for some roots (cliques, k = 4..6): pre_roots
+    // ascending (npre entries), root pre_roots[i]'s rows (d x NW uint64) and
+    // out-degrees (d int32) at pre + pre_offs[i]
+    const int32_t *pre_roots;
+    const int64_t *pre_offs;
+    const char *pre;
+    int32_t npre;
+    int32_t pad5;
 };
+
+// A static or donated job's query word: bits 0..15 the query index, bits
+// 16..23 the job's own donation threshold (0: the launch's steal_min).
+constexpr int kQueryBits = 16;
+constexpr int job_query(int32_t q) { return q & ((1 << kQueryBits) - 1); }
+constexpr int job_steal_min(int32_t q) { return (q >> kQueryBits) & 0xff; }
+constexpr int32_t make_job_query(int q, int steal_min) { return q | (steal_min << kQueryBits); }
 
 // kernels (enum_kernel.hip)
 // mode: 0 cliques only, 1 motifs only, 2 mixed (batch)
@@ -172,6 +187,13 @@ int enum_init_launch(const EnumParams &p, unsigned long long njobs, int queues, 
                      unsigned long long *results, size_t nresults, unsigned long long *done_ticks, int nq,
                      void *stream);
 int fill_u32_launch(void *ptr, uint32_t value, size_t count, void *stream);
+// builds the local bitmaps of p.pre_roots into p.pre (one block per root)
+int prebuild_launch(const EnumParams &p, void *stream);
+// bytes of a prebuilt bitmap of a root with out-degree d (0: not on the bitmap path)
+constexpr int64_t prebuilt_bytes(int32_t d) {
+    const int nw = d <= 64 ? 1 : d <= 128 ? 2 : d <= 256 ? 4 : 0;
+    return nw ? (((int64_t)d * nw * 8 + (int64_t)d * 4 + 15) & ~(int64_t)15) : 0;
+}
 
 }  // namespace dm
 

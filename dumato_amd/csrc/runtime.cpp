@@ -316,44 +316,6 @@ bool root_has_work(const dm_graph *g, int32_t v, int K, bool motif) {
     return g->rcsr.off[v + 1] - g->ostart[v] > 0;
 }
 
-std::vector<int32_t> order_by_cost(const std::vector<StaticJob> &jobs) {
-    const int B = 256;
-    double lmax = -1e300, lmin = 1e300;
-    for (const StaticJob &j : jobs) {
-        const double l = std::log(std::max(j.cost, 1e-300));
-        lmax = std::max(lmax, l);
-        lmin = std::min(lmin, l);
-    }
-    const double scale = lmax > lmin ? (B - 1) / (lmax - lmin) : 0.0;
-    std::vector<int32_t> bkt(jobs.size()), start(B + 1, 0), order(jobs.size());
-    for (size_t i = 0; i < jobs.size(); i++) {
-        const int b = (int)((lmax - std::log(std::max(jobs[i].cost, 1e-300))) * scale);
-        bkt[i] = std::min(std::max(b, 0), B - 1);
-        start[bkt[i] + 1]++;
-    }
-    for (int b = 0; b < B; b++) start[b + 1] += start[b];
-    for (size_t i = 0; i < jobs.size(); i++) order[start[bkt[i]]++] = (int32_t)i;
-    return order;
-}
-
-void write_queues(const std::vector<StaticJob> &jobs, const std::vector<std::vector<int32_t>> &q,
-                  std::vector<int64_t> &sq_off, std::vector<int32_t> &sq_jobs) {
-    const size_t workers = q.size();
-    sq_off.assign(workers + 1, 0);
-    for (size_t w = 0; w < workers; w++) sq_off[w + 1] = sq_off[w] + (int64_t)q[w].size();
-    sq_jobs.assign((size_t)sq_off[workers] * 4, 0);
-    for (size_t w = 0; w < workers; w++) {
-        int64_t o = sq_off[w];
-        for (int32_t ji : q[w]) {
-            sq_jobs[4 * o] = jobs[ji].a;
-            sq_jobs[4 * o + 1] = jobs[ji].b;
-            sq_jobs[4 * o + 2] = jobs[ji].c;
-            sq_jobs[4 * o + 3] = 0;
-            o++;
-        }
-    }
-}
-
 // Plain: one job per root, in descending degree order (out-degree for
 // cliques), dealt round-robin to the workers like the reference's
 // v -> warp v % W partition (src/main/DuMatoCPU.cu:121-135), heaviest first.
@@ -362,17 +324,31 @@ void write_queues(const std::vector<StaticJob> &jobs, const std::vector<std::vec
 void plain_partition(const dm_graph *g, const std::vector<int32_t> &roots, int K, bool motif,
                      int workers, std::vector<int64_t> &sq_off, std::vector<int32_t> &sq_jobs) {
     (void)K;
-    std::vector<StaticJob> jobs(roots.size());
-    for (size_t i = 0; i < roots.size(); i++) {
+    // descending (out-)degree by a counting sort (equal degrees keep their
+    // ascending rank), then job t to worker t mod workers
+    const size_t nr = roots.size(), W = (size_t)std::max(workers, 1);
+    std::vector<int32_t> key(nr);
+    int32_t kmax = 0;
+    for (size_t i = 0; i < nr; i++) {
         const int32_t v = roots[i];
-        const double d = motif ? (double)(g->rcsr.off[v + 1] - g->rcsr.off[v])
-                               : (double)(g->rcsr.off[v + 1] - g->ostart[v]);
-        jobs[i] = {v, 0, INT_MAX, 1.0 + d};
+        key[i] = (int32_t)(g->rcsr.off[v + 1] - (motif ? g->rcsr.off[v] : g->ostart[v]));
+        kmax = std::max(kmax, key[i]);
     }
-    const std::vector<int32_t> order = order_by_cost(jobs);
-    std::vector<std::vector<int32_t>> q((size_t)std::max(workers, 1));
-    for (size_t t = 0; t < order.size(); t++) q[t % q.size()].push_back(order[t]);
-    write_queues(jobs, q, sq_off, sq_jobs);
+    std::vector<int64_t> start((size_t)kmax + 2, 0);
+    for (size_t i = 0; i < nr; i++) start[(size_t)(kmax - key[i]) + 1]++;
+    for (size_t b = 1; b < start.size(); b++) start[b] += start[b - 1];
+    std::vector<int32_t> order(nr);
+    for (size_t i = 0; i < nr; i++) order[start[(size_t)(kmax - key[i])]++] = (int32_t)i;
+    sq_off.assign(W + 1, 0);
+    for (size_t w = 0; w < W; w++) sq_off[w + 1] = sq_off[w] + (int64_t)(nr > w ? (nr - 1 - w) / W + 1 : 0);
+    sq_jobs.assign(nr * 4, 0);
+    for (size_t t = 0; t < nr; t++) {
+        int32_t *jp = &sq_jobs[4 * (size_t)(sq_off[t % W] + (int64_t)(t / W))];
+        jp[0] = roots[order[t]];
+        jp[1] = 0;
+        jp[2] = INT_MAX;
+        jp[3] = 0;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -404,11 +380,12 @@ int run_enum(dm_graph *g, int K, bool motif, const dm_config &cfg, uint64_t *his
     // roots and their static partition (reused while nothing it depends on
     // changes)
     const double t_s0 = now_ms();
+    g->last_infer_ms = g->last_bin_ms = 0;
     int32_t rb = std::max(cfg.root_begin, 0), re = cfg.root_end > 0 ? std::min(cfg.root_end, n) : n;
     const int learned = cfg.sched == DM_SCHED_LEARNED ? 1 : 0;
     const int key[7] = {K, motif ? 1 : 0, learned, sh.workers, rb, re, sample_mod};
     dm_graph::Partition *part = nullptr;
-    for (size_t i = 0; i < g->partitions.size(); i++) {
+    for (size_t i = 0; i < g->partitions.size() && !cfg.repartition; i++) {
         dm_graph::Partition &c = g->partitions[i];
         if (std::equal(key, key + 7, c.key) && (!learned || c.model_gen == g->model_gen)) {
             if (i) std::swap(g->partitions[0], c);
@@ -432,10 +409,15 @@ int run_enum(dm_graph *g, int K, bool motif, const dm_config &cfg, uint64_t *his
         np.model_gen = g->model_gen;
         np.skipped_bytes = skipped_bytes;
         if (learned) {
-            if ((r = learned_partition(g, roots, K, motif, cfg, sh, np.sq_off, np.sq_jobs))) return r;
+            if ((r = learned_partition(g, roots, K, motif, cfg, sh, np))) return r;
         } else {
             plain_partition(g, roots, K, motif, sh.workers, np.sq_off, np.sq_jobs);
         }
+        g->partitions.erase(std::remove_if(g->partitions.begin(), g->partitions.end(),
+                                           [&](const dm_graph::Partition &c) {
+                                               return std::equal(key, key + 7, c.key);
+                                           }),
+                            g->partitions.end());
         if (g->partitions.size() >= 8) g->partitions.pop_back();
         g->partitions.insert(g->partitions.begin(), std::move(np));
         part = &g->partitions[0];
@@ -443,6 +425,7 @@ int run_enum(dm_graph *g, int K, bool motif, const dm_config &cfg, uint64_t *his
     const std::vector<int64_t> &sq_off = part->sq_off;
     const std::vector<int32_t> &sq_jobs = part->sq_jobs;
     const uint64_t skipped_bytes = part->skipped_bytes;
+    const int npre = cfg.clique_bitmap ? (int)part->pre_roots.size() : 0;
     const double t_s1 = now_ms();
     const int64_t njobs = (int64_t)sq_jobs.size() / 4;
 
@@ -468,8 +451,18 @@ int run_enum(dm_graph *g, int K, bool motif, const dm_config &cfg, uint64_t *his
     qd.lut = lut;
     qd.out = d->d_results;
     if ((r = dev_grow(d->d_qd, d->qd_cap, 1))) return r;
+    if (npre > 0) {
+        if ((r = dev_grow(d->d_pre_roots, d->pre_roots_cap, (size_t)npre)) ||
+            (r = dev_grow(d->d_pre_offs, d->pre_offs_cap, (size_t)npre)) ||
+            (r = dev_grow(d->d_pre, d->pre_cap, (size_t)part->pre_bytes)))
+            return r;
+    }
 
     hipStream_t s = d->stream;
+    if (npre > 0) {
+        HIPCHK(hipMemcpyAsync(d->d_pre_roots, part->pre_roots.data(), (size_t)npre * 4, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(d->d_pre_offs, part->pre_offs.data(), (size_t)npre * 8, hipMemcpyHostToDevice, s));
+    }
     HIPCHK(hipMemcpyAsync(d->d_qd, &qd, sizeof(qd), hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync(d->d_sq_off, sq_off.data(), sq_off.size() * 8, hipMemcpyHostToDevice, s));
     if (!sq_jobs.empty())
@@ -506,12 +499,18 @@ int run_enum(dm_graph *g, int K, bool motif, const dm_config &cfg, uint64_t *his
     p.poll_ticks = (uint64_t)cfg.steal_period * (uint64_t)d->wall_khz / 1000;
     p.bitmap = cfg.clique_bitmap ? 1 : 0;
     p.watchdog_ticks = cfg.watchdog_s > 0 ? (uint64_t)(cfg.watchdog_s * d->wall_khz * 1000.0) : 0;
+    p.npre = npre;
+    p.pre_roots = npre ? d->d_pre_roots : nullptr;
+    p.pre_offs = npre ? d->d_pre_offs : nullptr;
+    p.pre = npre ? d->d_pre : nullptr;
     int queues = 0;
     for (size_t w = 0; w + 1 < sq_off.size(); w++) queues += sq_off[w + 1] > sq_off[w] ? 1 : 0;
     if ((r = enum_init_launch(p, (unsigned long long)njobs, queues, d->d_epoch, d->d_results, nres, nullptr, 0, s)))
         return r;
 
+    // the kernel time includes the bitmap pre-pass of a learned partition
     HIPCHK(hipEventRecord(d->ev0, s));
+    if (npre > 0 && (r = prebuild_launch(p, s))) return r;
     if (njobs > 0 && (r = enum_launch(p, motif ? 1 : 0, sh.vwave, sh.blocks, sh.threads, sh.lds_total, s))) return r;
     HIPCHK(hipEventRecord(d->ev1, s));
     SchedCounters sc1;
@@ -554,6 +553,8 @@ int run_enum(dm_graph *g, int K, bool motif, const dm_config &cfg, uint64_t *his
     memset(&st, 0, sizeof(st));
     st.device_ms = kms;
     st.sched_ms = t_s1 - t_s0;
+    st.infer_ms = g->last_infer_ms;
+    st.bin_ms = g->last_bin_ms;
     st.wall_ms = now_ms() - t_wall0;
     st.subgraphs = total;
     st.alg_bytes = sc1.alg_bytes + skipped_bytes;

@@ -47,6 +47,13 @@ struct DeviceCtx {
     size_t results_cap = 0;
     QueryDesc *d_qd = nullptr;
     size_t qd_cap = 0;
+    // prebuilt local bitmaps of a learned partition's heavy roots
+    int32_t *d_pre_roots = nullptr;
+    size_t pre_roots_cap = 0;
+    int64_t *d_pre_offs = nullptr;
+    size_t pre_offs_cap = 0;
+    char *d_pre = nullptr;
+    size_t pre_cap = 0;
 };
 
 // The current device's context (created, with its stream, events and queue,
@@ -99,9 +106,15 @@ struct dm_graph {
         std::vector<int64_t> sq_off;
         std::vector<int32_t> sq_jobs;
         uint64_t skipped_bytes;
+        // roots whose local bitmaps a pre-pass builds once for their pieces
+        // (ascending), their offsets in the prebuilt buffer, its size
+        std::vector<int32_t> pre_roots;
+        std::vector<int64_t> pre_offs;
+        int64_t pre_bytes = 0;
     };
     std::vector<Partition> partitions;  // most recent first
     uint64_t model_gen = 0;
+    double last_infer_ms = 0, last_bin_ms = 0;  // phases of the last learned partition
 };
 
 namespace dm {
@@ -134,18 +147,6 @@ int pick_shape(dm_graph *g, int K, bool motif, const dm_config &cfg, Shape &sh);
 bool root_has_work(const dm_graph *g, int32_t v, int K, bool motif);
 void plain_partition(const dm_graph *g, const std::vector<int32_t> &roots, int K, bool motif, int workers,
                      std::vector<int64_t> &sq_off, std::vector<int32_t> &sq_jobs);
-// A static job: (root, lo, hi) children of a root, or a range of roots
-// (r0, -1, r1); cost in the partition's own units.
-struct StaticJob {
-    int32_t a, b, c;
-    double cost;
-};
-// Indices of jobs in descending cost order (log-bucketed: ties within a
-// bucket keep their input order).
-std::vector<int32_t> order_by_cost(const std::vector<StaticJob> &jobs);
-// Writes per-worker queues: queue w holds the jobs listed in q[w].
-void write_queues(const std::vector<StaticJob> &jobs, const std::vector<std::vector<int32_t>> &q,
-                  std::vector<int64_t> &sq_off, std::vector<int32_t> &sq_jobs);
 int run_enum(dm_graph *g, int K, bool motif, const dm_config &cfg, uint64_t *hist, size_t hist_len,
              uint64_t *count_out, int sample_mod = 1);
 int check_config(const dm_config &cfg);
@@ -158,7 +159,7 @@ int device_error(int code, const char *what);
 // sched.cpp
 int autotuned_vwave(dm_graph *g, int K, bool motif);
 int learned_partition(dm_graph *g, const std::vector<int32_t> &roots, int K, bool motif, const dm_config &cfg,
-                      const Shape &sh, std::vector<int64_t> &sq_off, std::vector<int32_t> &sq_jobs);
+                      const Shape &sh, dm_graph::Partition &part);
 void free_model(dm_graph *g);
 
 // graph.cpp

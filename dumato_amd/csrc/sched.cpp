@@ -386,84 +386,177 @@ static int predict_all(dm_graph *g, std::vector<float> &pred) {
 }
 
 // ---------------------------------------------------------------------------
-// learned partition: order by predicted cost, split predicted-heavy roots into
-// child ranges, deal pieces to the least-loaded worker (LPT).
+// Learned partition.  The model's predicted cost ranks the roots (it is
+// trained on the costs of a smaller k, so its ranking, not its magnitude, is
+// what carries over to the queried k):
+//  * the H = max(64, 2%) highest-ranked roots are cut into pieces of equal
+//    estimated work by first-level child ranges, 64 pieces for the top 32,
+//    32 for the next 96, 16 for the next 384, 8 for the rest, and each piece
+//    donates from as few as 2 children left; on the clique bitmap path
+//    (k = 4..6, out-degree <= 256) a pre-pass builds their local bitmaps
+//    once, so a piece copies its rows instead of rebuilding them;
+//  * the lower half of the roots keeps 16 children before it donates (a
+//    light job is not worth rebuilding its prefix elsewhere); the rest keep
+//    the launch's threshold.
+// The heaviest 2 x workers jobs go longest-processing-time first to the
+// least-loaded worker; the rest are dealt in snake order by predicted cost.
+// Everything but the head is bucketed, so the host cost stays linear.
 int learned_partition(dm_graph *g, const std::vector<int32_t> &roots, int K, bool motif, const dm_config &cfg,
-                      const Shape &sh, std::vector<int64_t> &sq_off, std::vector<int32_t> &sq_jobs) {
-    (void)cfg;
+                      const Shape &sh, dm_graph::Partition &part) {
     if (!g->model) return set_error(DM_ERR_ARG, "learned scheduler: train a model first (dm_sched_train)");
+    using clk = std::chrono::steady_clock;
+    const auto t0 = clk::now();
     int r = compute_features(g);
     if (r) return r;
     std::vector<float> pred;
     if ((r = predict_all(g, pred))) return r;
-    // predicted cost per root; roots above half a worker's share are cut
-    // into child ranges
-    std::vector<double> cost(roots.size());
-    double total = 0;
-    for (size_t i = 0; i < roots.size(); i++) total += (cost[i] = std::exp((double)pred[roots[i]]));
-    const int workers = sh.workers;
-    const double target = total / std::max(workers, 1);
+    const auto t1 = clk::now();
+    g->last_infer_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    struct BinTimer {
+        dm_graph *g;
+        clk::time_point t;
+        ~BinTimer() { g->last_bin_ms = std::chrono::duration<double, std::milli>(clk::now() - t).count(); }
+    } bin_timer{g, t1};
+
+    struct Job {
+        int32_t root, lo, hi, q;
+        double cost;
+    };
+    const size_t nr = roots.size();
+    const size_t W = (size_t)std::max(sh.workers, 1);
+    const size_t H = std::min(nr, std::max<size_t>(64, nr / 50));
+    auto key = [&](int32_t i) { return pred[roots[i]]; };
+    std::vector<int32_t> idx(nr);
+    std::iota(idx.begin(), idx.end(), 0);
+    auto desc = [&](int32_t a, int32_t b) { return key(a) > key(b) || (key(a) == key(b) && a < b); };
+    std::nth_element(idx.begin(), idx.begin() + H, idx.end(), desc);
+    std::sort(idx.begin(), idx.begin() + H, desc);
+    const bool halves = nr > 2 * H;
+    if (halves) std::nth_element(idx.begin() + H, idx.begin() + nr / 2, idx.end(), desc);
+
     const bool can_split = !(K == 2 && !motif);
-    std::vector<StaticJob> jobs;
-    jobs.reserve(roots.size() + 1024);
-    for (size_t ri = 0; ri < roots.size(); ri++) {
-        const int32_t v = roots[ri];
-        const double c = cost[ri];
+    const bool bitmap_path = !motif && cfg.clique_bitmap && K >= 4 && K <= 6;
+    const int e = motif ? 1 : std::max(1, K - 3);
+    auto wt = [e](int m) {  // m^e
+        double x = 1.0;
+        for (int i = 0; i < e; i++) x *= (double)m;
+        return x;
+    };
+    std::vector<Job> jobs;
+    jobs.reserve(nr + H * 16);
+    part.pre_roots.clear();
+    for (size_t rk = 0; rk < H; rk++) {
+        const int32_t v = roots[idx[rk]];
+        const double c = std::exp((double)key(idx[rk]));
         const int nch = (int)(g->rcsr.off[v + 1] - g->ostart[v]);  // children of the root
-        int parts = 1;
-        if (can_split && c > 0.5 * target && nch >= 2)
-            parts = (int)std::min<double>(nch, std::ceil(c / (0.5 * target)));
-        if (parts <= 1) {
-            jobs.push_back({v, 0, INT_MAX, c});
+        int P = rk < 32 ? 64 : rk < 128 ? 32 : rk < 512 ? 16 : 8;
+        P = can_split ? std::min(P, nch) : 1;
+        const int32_t q = make_job_query(0, 2);
+        if (P <= 1) {
+            jobs.push_back({v, 0, INT_MAX, q, c});
             continue;
         }
-        // child i of a root carries roughly (nch - i)^e of its work (later
+        // child i carries about (nch - i)^e of the root's work (later
         // children see fewer candidates above them)
-        const double e = motif ? 1.0 : (double)std::max(1, K - 3);
-        std::vector<double> w(nch);
         double ws = 0;
-        for (int i = 0; i < nch; i++) ws += (w[i] = std::pow((double)(nch - i), e));
+        for (int i = 0; i < nch; i++) ws += wt(nch - i);
         double acc = 0;
         int lo = 0;
-        for (int pidx = 1; pidx <= parts && lo < nch; pidx++) {
-            const double goal = ws * pidx / parts;
+        for (int pi = 1; pi <= P && lo < nch; pi++) {
+            const double goal = ws * pi / P;
             int hi = lo;
             double a = acc;
-            while (hi < nch && (a < goal || hi == lo)) a += w[hi++];
-            if (pidx == parts) {
-                while (hi < nch) a += w[hi++];
-            }
-            jobs.push_back({v, lo, hi >= nch ? INT_MAX : hi, c * (a - acc) / ws});
+            while (hi < nch && (a < goal || hi == lo)) a += wt(nch - hi++);
+            if (pi == P)
+                while (hi < nch) a += wt(nch - hi++);
+            jobs.push_back({v, lo, hi >= nch ? INT_MAX : hi, q, c * (a - acc) / ws});
             acc = a;
             lo = hi;
         }
+        if (bitmap_path && nch <= 256) part.pre_roots.push_back(v);
     }
-    const std::vector<int32_t> order = order_by_cost(jobs);
-    // LPT (least-loaded worker first) for the heaviest jobs, then a snake
-    // deal of the light tail
-    using QE = std::pair<double, int>;
+    const size_t nheavy = jobs.size();
+    // the other roots in descending predicted cost: 256 buckets of the
+    // predicted log cost
+    float pmin = 1e30f, pmax = -1e30f;
+    for (size_t t = H; t < nr; t++) {
+        pmin = std::min(pmin, key(idx[t]));
+        pmax = std::max(pmax, key(idx[t]));
+    }
+    const float scale = pmax > pmin ? 255.f / (pmax - pmin) : 0.f;
+    std::vector<int32_t> bstart(257, 0);
+    auto bucket = [&](int32_t i) { return std::min(255, std::max(0, (int)((pmax - key(i)) * scale))); };
+    for (size_t t = H; t < nr; t++) bstart[bucket(idx[t]) + 1]++;
+    for (int b = 0; b < 256; b++) bstart[b + 1] += bstart[b];
+    std::vector<int32_t> tail(nr - H);
+    for (size_t t = H; t < nr; t++) tail[bstart[bucket(idx[t])]++] = idx[t];
+    std::vector<char> light(nr, 0);
+    if (halves)
+        for (size_t t = nr / 2; t < nr; t++) light[idx[t]] = 1;
+    for (int32_t i : tail) {
+        const int32_t q = light[i] ? make_job_query(0, 16) : 0;
+        jobs.push_back({roots[i], 0, INT_MAX, q, 0.0});
+    }
+
+    // deal: LPT over the heaviest jobs (the pieces and the highest-ranked
+    // roots, already in rank order), then snake over the least-loaded order
+    const size_t J = jobs.size();
+    std::vector<int32_t> owner(J);
+    const size_t head = std::min(nheavy, 2 * W);
+    std::vector<size_t> hord(nheavy);
+    std::iota(hord.begin(), hord.end(), 0);
+    std::partial_sort(hord.begin(), hord.begin() + head, hord.end(),
+                      [&](size_t a, size_t b) { return jobs[a].cost > jobs[b].cost; });
+    using QE = std::pair<double, int32_t>;
     std::priority_queue<QE, std::vector<QE>, std::greater<QE>> heap;
-    for (int w = 0; w < workers; w++) heap.push({0.0, w});
-    std::vector<std::vector<int32_t>> q((size_t)std::max(workers, 1));
-    const size_t head = std::min(order.size(), (size_t)workers * 2);
+    for (size_t w = 0; w < W; w++) heap.push({0.0, (int32_t)w});
     for (size_t t = 0; t < head; t++) {
-        QE e = heap.top();
+        QE x = heap.top();
         heap.pop();
-        q[e.second].push_back(order[t]);
-        e.first += jobs[order[t]].cost;
-        heap.push(e);
+        owner[hord[t]] = x.second;
+        x.first += jobs[hord[t]].cost;
+        heap.push(x);
     }
-    std::vector<int> by_load(workers);
-    for (int w = 0; w < workers; w++) {
+    std::vector<int32_t> by_load(W);
+    for (size_t w = 0; w < W; w++) {
         by_load[w] = heap.top().second;  // least loaded first
         heap.pop();
     }
-    for (size_t t = head; t < order.size(); t++) {
-        size_t i = t - head, lap = i / workers, pos = i % workers;
-        int w = by_load[(lap & 1) ? workers - 1 - pos : pos];
-        q[w].push_back(order[t]);
+    size_t s = 0;
+    auto deal = [&](size_t j) {
+        const size_t lap = s / W, pos = s % W;
+        owner[j] = by_load[(lap & 1) ? W - 1 - pos : pos];
+        s++;
+    };
+    for (size_t t = head; t < nheavy; t++) deal(hord[t]);
+    for (size_t j = nheavy; j < J; j++) deal(j);
+
+    // queues, in dealing order within each worker (LPT head first)
+    part.sq_off.assign(W + 1, 0);
+    for (size_t j = 0; j < J; j++) part.sq_off[(size_t)owner[j] + 1]++;
+    for (size_t w = 0; w < W; w++) part.sq_off[w + 1] += part.sq_off[w];
+    std::vector<int64_t> fill(part.sq_off.begin(), part.sq_off.end() - 1);
+    part.sq_jobs.assign(J * 4, 0);
+    auto put = [&](size_t j) {
+        int32_t *jp = &part.sq_jobs[4 * (size_t)fill[owner[j]]++];
+        jp[0] = jobs[j].root;
+        jp[1] = jobs[j].lo;
+        jp[2] = jobs[j].hi;
+        jp[3] = jobs[j].q;
+    };
+    for (size_t t = 0; t < nheavy; t++) put(hord[t]);
+    for (size_t j = nheavy; j < J; j++) put(j);
+
+    // prebuilt bitmaps, ascending by root
+    std::sort(part.pre_roots.begin(), part.pre_roots.end());
+    part.pre_offs.resize(part.pre_roots.size());
+    int64_t off = 0;
+    for (size_t i = 0; i < part.pre_roots.size(); i++) {
+        const int32_t v = part.pre_roots[i];
+        part.pre_offs[i] = off;
+        off += prebuilt_bytes((int32_t)(g->rcsr.off[v + 1] - g->ostart[v]));
     }
-    write_queues(jobs, q, sq_off, sq_jobs);
+    part.pre_bytes = std::max<int64_t>(off, 16);
     return DM_OK;
 }
 

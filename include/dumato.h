@@ -261,9 +261,18 @@ typedef struct dm_dfs_data {
     const int32_t *stop;      /* host sets 1: every worker leaves its loop at the next step */
 } dm_dfs_data;
 
-#define DM_DFS_CLIQUE_PA 0 /* clique_PA.cu: extend_clique / forward / backward_clique */
-#define DM_DFS_CLIQUE_WC 1 /* clique_DM_WCV8.cu: extend / filterClique / forward / backward */
-#define DM_DFS_MOTIF 2     /* motifs_DM_WCLB.cu: extend(0,k) / canonicalFilter2 / forward_induce */
+#define DM_DFS_CLIQUE_PA 0   /* clique_PA.cu: extend_clique / forward / backward_clique */
+#define DM_DFS_CLIQUE_WC 1   /* clique_DM_WCV8.cu: extend / filterClique / forward / backward */
+#define DM_DFS_MOTIF 2       /* motifs_DM_WCLB.cu: extend(0,k) / canonicalFilter2 / forward_induce */
+#define DM_DFS_NEAR_CLIQUE 3 /* q4_PA.cu .. q13_PA.cu: extend_q / forward / backward_q */
+
+/* What a step-API state enumerates (dm_dfs_create). */
+#define DM_PATTERN_CLIQUE 0      /* k-cliques: clique_PA / clique_DM_WCV kernels */
+#define DM_PATTERN_MOTIF 1       /* k-motif histogram: motifs kernel */
+#define DM_PATTERN_NEAR_CLIQUE 2 /* the pattern-aware query qk (4 <= k <= 13): a
+                                    (k-2)-clique and two vertices adjacent to all of it,
+                                    counted per clique and unordered pair of common
+                                    neighbours (K_k minus one edge, not induced) */
 
 typedef struct dm_dfs_config {
     int32_t vwave;           /* lanes per DFS worker: 8, 16, 32, 64 (0 = 32, the reference's warp) */
@@ -288,10 +297,10 @@ typedef struct dm_dfs_stats {
 
 typedef struct dm_dfs dm_dfs;
 int dm_dfs_config_init(dm_dfs_config *cfg);
-/* Device state for k-clique (motif = 0) or k-motif (motif = 1) enumeration
- * of g with the step API (reference: the DuMatoCPU constructor,
+/* Device state for enumerating the DM_PATTERN_* `pattern` of size k in g with
+ * the step API (reference: the DuMatoCPU constructor,
  * src/main/DuMatoCPU.cu:13-30, 204-276). */
-int dm_dfs_create(dm_graph *g, int k, int motif, const dm_dfs_config *cfg, dm_dfs **out);
+int dm_dfs_create(dm_graph *g, int k, int pattern, const dm_dfs_config *cfg, dm_dfs **out);
 /* Host stub of a kernel of this library written against the step API
  * (kind = DM_DFS_*) for the given width, for dm_dfs_run. */
 int dm_dfs_kernel(int kind, int vwave, const void **kernel);

@@ -60,6 +60,8 @@ class Worker {
     __device__ void start(const dm_dfs_data &dd) { init(dd, kWC); }
     __device__ void start_clique(const dm_dfs_data &dd) { init(dd, kPA); }
     __device__ void start_induce(const dm_dfs_data &dd) { init(dd, kMotif); }
+    // pattern-aware near-clique query (start_q4 .. start_q13 with K = 4..13)
+    __device__ void start_q(const dm_dfs_data &dd) { init(dd, kQ); }
 
     // Saves the level (the resume point) and adds the worker's counts to the
     // result.  Synchronises the block.
@@ -151,6 +153,15 @@ class Worker {
         set_count(k_, w, true);
     }
 
+    // pattern-aware near-clique query (extend_q4 .. extend_q13,
+    // src/main/DuMatoGPU.cu:1222-1468): a K-vertex pattern of a
+    // (K-2)-clique and two vertices adjacent to all of it.  Levels 0..K-4
+    // build the clique like extend_clique; level K-3 is the common
+    // neighbourhood of the clique (any id, members excluded); level K-2 the
+    // entries of level K-3 above the vertex taken there, so the last level
+    // counts the unordered pairs.
+    __device__ void extend_q() { extend_q_at(k_, true); }
+
     // ---- filter ----
     // clique (DM_WCV filterClique): keep the extensions adjacent to id[0..k-1]
     __device__ void filterClique() {
@@ -231,6 +242,7 @@ class Worker {
     __device__ void backward() { up(kWC); }
     __device__ void backward_clique() { up(kPA); }
     __device__ void backward_induce() { up(kMotif); }
+    __device__ void backward_q() { up(kQ); }
 
     // ---- lane primitives of the worker ----
     __device__ unsigned long long ballot(bool p) const {
@@ -242,7 +254,7 @@ class Worker {
     __device__ int below(unsigned long long m) const { return __popcll(m & ((1ull << lane_) - 1)); }
 
    private:
-    enum { kWC = 0, kPA = 1, kMotif = 2 };
+    enum { kWC = 0, kPA = 1, kMotif = 2, kQ = 3 };
     dm_dfs_data d_;
     int wid_, lane_, k_;
     bool ok_;  // a worker of the state (and the state's width)
@@ -308,6 +320,40 @@ class Worker {
         set_count(l, w, set_next);
     }
 
+    __device__ void extend_q_at(int l, bool set_next) {
+        const int K = d_.K;
+        if (l <= K - 4) {  // the clique levels
+            extend_clique_at(l, set_next);
+            return;
+        }
+        int32_t *out = list(l);
+        int w = 0;
+        if (l == K - 3) {
+            // N(id[0]) & ... & N(id[l]); a member is in no list of its own
+            const int32_t v0 = getId(0);
+            const int32_t *nb = getAdjacency(v0);
+            const int dv = degree(v0);
+            for (int base = 0; base < dv; base += W) {
+                const int t = base + lane_;
+                const int32_t x = t < dv ? nb[t] : -1;
+                bool ok = t < dv;
+                for (int i = 1; i <= l && ok; i++) ok = adjacent(getId(i), x);
+                w += write(out, w, x, ok);
+            }
+        } else {
+            // the previous level's full list above the vertex taken from it
+            const int32_t last = getId(l);
+            const int32_t *prev = list(l - 1);
+            const int np = row(d_.fixed)[l - 1];
+            for (int base = 0; base < np; base += W) {
+                const int t = base + lane_;
+                const int32_t x = t < np ? prev[t] : -1;
+                w += write(out, w, x, t < np && x > last);
+            }
+        }
+        set_count(l, w, set_next);
+    }
+
     __device__ int32_t step_down() {
         const int nx = getCurrentNumberOfExtensions();
         const int32_t x = list(k_)[nx - 1];
@@ -357,6 +403,8 @@ class Worker {
         k_ = L;
         if (mode == kPA)
             for (int l = 0; l < L; l++) extend_clique_at(l, false);
+        if (mode == kQ)
+            for (int l = 0; l < L; l++) extend_q_at(l, false);
     }
 
     __device__ void init(const dm_dfs_data &dd, int mode) {

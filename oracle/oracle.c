@@ -34,7 +34,12 @@
  *       Pinned by the brute force below (which never uses the canonical
  *       rule), closed forms on stars, paths, cycles and complete graphs,
  *       sum(hist) = number of connected k-subsets, and known answers.
- *   or_brute_clique / or_brute_motif
+ *   or_near_clique_count
+ *       the pattern-aware queries q4_PA .. q13_PA: extend_q4 .. extend_q13
+ *       (src/main/DuMatoGPU.cu:1222-1468).  Pinned by or_brute_near_clique
+ *       (a count per k-subset by its missing edges, no DFS) and the closed
+ *       form C(n,k)*C(k,2) on complete graphs.
+ *   or_brute_clique / or_brute_motif / or_brute_near_clique
  *       Enumerate all k-subsets; pinned against a pure-Python brute force
  *       and closed forms.
  *   or_clique_bytes / or_motif_bytes
@@ -305,6 +310,69 @@ uint64_t or_clique_count(int32_t n, const int64_t *off, const int32_t *adj,
         if (per_root) per_root[v] = c;
         total += c;
     }
+    return total;
+}
+
+/* ---- near-clique query (q4_PA .. q13_PA) ----------------------------------
+ * DuMatoGPU::extend_q4 .. extend_q13 (src/main/DuMatoGPU.cu:1222-1468) with
+ * extend_single / copy_extensions (:1028-1057) and intersect /
+ * intersect_unique (:353-383): levels 0..k-4 grow a clique in ascending ids
+ * (neighbours of the last vertex above it, kept if adjacent to every earlier
+ * one); level k-3 is the common neighbourhood of the (k-2)-clique (any id,
+ * members excluded); level k-2 keeps the entries of level k-3 above the
+ * vertex taken there, and the last level adds their number
+ * (aggregate_counter).  So each clique adds C(c, 2) for its c common
+ * neighbours. */
+static uint64_t near_clique_root(const int64_t *off, const int32_t *adj,
+                                 int k, int faithful, int32_t root) {
+    int32_t id[OR_MAXK];
+    int64_t cur[OR_MAXK];
+    uint64_t count = 0;
+    const int top = k - 3; /* depth at which id[0..k-3] is the clique */
+    int d = 0;
+    id[0] = root;
+    cur[0] = off[root];
+    while (d >= 0) {
+        if (d == top) {
+            /* common neighbours of id[0..top], then the pairs among them */
+            uint64_t c = 0;
+            for (int64_t p = off[id[0]]; p < off[id[0] + 1]; p++) {
+                int32_t x = adj[p];
+                int ok = 1;
+                for (int i = 1; i <= top && ok; i++)
+                    ok = x != id[i] && has_nbr(off, adj, id[i], x, faithful);
+                c += ok;
+            }
+            count += c * (c - (c > 0)) / 2;
+            d--;
+            continue;
+        }
+        if (cur[d] >= off[id[d] + 1]) {
+            d--;
+            continue;
+        }
+        int32_t ext = adj[cur[d]++];
+        if (ext <= id[d]) continue;
+        int ok = 1;
+        for (int i = 0; i < d && ok; i++)
+            ok = has_nbr(off, adj, id[i], ext, faithful);
+        if (!ok) continue;
+        d++;
+        id[d] = ext;
+        cur[d] = off[ext];
+    }
+    return count;
+}
+
+uint64_t or_near_clique_count(int32_t n, const int64_t *off,
+                              const int32_t *adj, int k, int nthreads,
+                              int faithful) {
+    if (k < 4 || k > OR_MAXK) return UINT64_MAX;
+    set_threads(nthreads);
+    uint64_t total = 0;
+#pragma omp parallel for schedule(dynamic, 16) reduction(+ : total)
+    for (int32_t v = 0; v < n; v++)
+        total += near_clique_root(off, adj, k, faithful, v);
     return total;
 }
 
@@ -673,4 +741,26 @@ uint64_t or_brute_motif(int32_t n, const int64_t *off, const int32_t *adj,
         total++;
     } while (next_comb(c, k, n));
     return total;
+}
+
+/* Brute force: each k-subset adds C(k,2) when it is a clique and 1 when
+ * exactly one of its vertex pairs is not an edge (the pair is then the
+ * only choice of the two common neighbours). */
+uint64_t or_brute_near_clique(int32_t n, const int64_t *off,
+                              const int32_t *adj, int k) {
+    uint64_t rows[64];
+    if (k < 4 || k > n) return 0;
+    if (brute_setup(n, off, adj, rows)) return UINT64_MAX;
+    int c[OR_MAXK];
+    for (int i = 0; i < k; i++) c[i] = i;
+    uint64_t cnt = 0;
+    do {
+        int missing = 0;
+        for (int i = 0; i < k && missing < 2; i++)
+            for (int j = i + 1; j < k && missing < 2; j++)
+                missing += !((rows[c[i]] >> c[j]) & 1);
+        if (missing == 0) cnt += (uint64_t)k * (k - 1) / 2;
+        else if (missing == 1) cnt += 1;
+    } while (next_comb(c, k, n));
+    return cnt;
 }

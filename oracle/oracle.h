@@ -67,7 +67,17 @@ uint64_t or_motif_count_roots(int32_t n, const int64_t *off,
                               const int32_t *roots, int64_t nroots,
                               uint64_t *hist, uint64_t *per_root_out);
 
+/* Pattern-aware near-clique query qk (q4_PA .. q13_PA): sum over ascending
+ * (k-2)-cliques of C(c, 2), c = number of common neighbours of the clique.
+ * 4 <= k <= 16; UINT64_MAX on a bad k. */
+uint64_t or_near_clique_count(int32_t n, const int64_t *off,
+                              const int32_t *adj, int k, int nthreads,
+                              int faithful);
+
 /* ---- brute force over vertex subsets (tiny graphs only) ---------------- */
+/* qk by k-subsets: C(k,2) per k-clique, 1 per k-subset missing one edge. */
+uint64_t or_brute_near_clique(int32_t n, const int64_t *off,
+                              const int32_t *adj, int k);
 uint64_t or_brute_clique(int32_t n, const int64_t *off, const int32_t *adj,
                          int k);
 uint64_t or_brute_motif(int32_t n, const int64_t *off, const int32_t *adj,

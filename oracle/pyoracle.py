@@ -63,6 +63,10 @@ def lib():
         L.or_brute_clique.argtypes = [ctypes.c_int32, i64p, i32p, ctypes.c_int]
         L.or_brute_motif.restype = ctypes.c_uint64
         L.or_brute_motif.argtypes = [ctypes.c_int32, i64p, i32p, ctypes.c_int, u16p, ctypes.c_int, u64p]
+        L.or_near_clique_count.restype = ctypes.c_uint64
+        L.or_near_clique_count.argtypes = [ctypes.c_int32, i64p, i32p, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.or_brute_near_clique.restype = ctypes.c_uint64
+        L.or_brute_near_clique.argtypes = [ctypes.c_int32, i64p, i32p, ctypes.c_int]
         L.or_clique_bytes.restype = ctypes.c_uint64
         L.or_clique_bytes.argtypes = [ctypes.c_int32, i64p, i32p, ctypes.c_int, ctypes.c_int]
         L.or_motif_bytes.restype = ctypes.c_uint64
@@ -201,6 +205,19 @@ def motif_count_roots(g, k, roots, nthreads=0, faithful=False, table=None):
                                nthreads, 1 if faithful else 0, _ptr(roots, i32p), len(roots),
                                _ptr(hist, u64p), _ptr(pr, u64p))
     return int(r), hist, pr[:len(roots)]
+
+
+def near_clique_count(g, k, nthreads=0, faithful=False):
+    """The pattern-aware query qk (q4_PA .. q13_PA)."""
+    r = lib().or_near_clique_count(g.n, _ptr(g.off, i64p), _ptr(g.adj, i32p), int(k), nthreads,
+                                   1 if faithful else 0)
+    if r == U64_MAX:
+        raise ValueError("bad k")
+    return int(r)
+
+
+def brute_near_clique(g, k):
+    return int(lib().or_brute_near_clique(g.n, _ptr(g.off, i64p), _ptr(g.adj, i32p), int(k)))
 
 
 def brute_clique(g, k):

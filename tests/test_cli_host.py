@@ -16,7 +16,7 @@ def _run(tool, *args):
                           timeout=60)
 
 
-@pytest.mark.parametrize("tool", ["clique_PA", "motifs"])
+@pytest.mark.parametrize("tool", ["clique_PA", "motifs", "q4_PA", "q13_PA"])
 def test_wrong_argc_prints_usage(tool):
     r = _run(tool, "g.edgelist", 4)
     assert r.returncode == 1
@@ -24,7 +24,7 @@ def test_wrong_argc_prints_usage(tool):
     assert "Wrong amount of parameters!" in r.stdout
 
 
-@pytest.mark.parametrize("tool", ["clique_PA", "motifs"])
+@pytest.mark.parametrize("tool", ["clique_PA", "motifs", "q5_PA"])
 def test_missing_graph_file(tool, tmp_path):
     r = _run(tool, tmp_path / "absent.edgelist", 4, 102400, 256, 16, 30)
     assert r.returncode == 1

@@ -12,6 +12,7 @@ import pytest
 import dumato_amd as dm
 from dumato_amd import workloads as W
 from oracle import pyoracle as O
+from tests import graphs as G
 
 pytestmark = pytest.mark.gpu
 
@@ -89,6 +90,39 @@ def test_idle_balancing_small_graph_every_kind(citeseer):
         wt, wh = O.motif_count(ref, k)
         tot, hist, _ = dm.Dfs(g, k, True, cfg).run()
         assert tot == wt and np.array_equal(hist, wh)
+
+
+@pytest.mark.parametrize("vw", [16, 64])
+def test_near_clique_queries_match_oracle(citeseer, vw):
+    # q4_PA .. q13_PA: the near-clique pattern of each size, without and with
+    # host-driven balancing
+    g, ref = citeseer
+    for k in (4, 5, 6, 7):
+        want = O.near_clique_count(ref, k)
+        for thr in (100, 0):
+            print("case dfs near-clique k=%d vwave %d threshold %d" % (k, vw, thr), flush=True)
+            cfg = dm.dfs_config(vwave=vw, threshold=thr, poll_us=100)
+            assert dm.Dfs(g, k, dm.DM_PATTERN_NEAR_CLIQUE, cfg).run()[0] == want
+    # dense enough that the largest queries still have occurrences
+    s, d, n = G.gnp(28, 0.85, 5)
+    dense = dm.Graph.from_edges(s, d, n)
+    dref = O.csr_from_edges(s, d, n)
+    assert O.near_clique_count(dref, 13) > 0
+    for k in range(4, 14):
+        print("case dfs near-clique dense k=%d vwave %d" % (k, vw), flush=True)
+        cfg = dm.dfs_config(vwave=vw, threshold=100)
+        assert dm.Dfs(dense, k, dm.DM_PATTERN_NEAR_CLIQUE, cfg).run()[0] == O.near_clique_count(dref, k)
+
+
+def test_near_clique_forced_preemption_rmat():
+    g = W.rmat_s17()
+    want = WORK["rmat_s17"]["near_clique"]["5"]
+    print("case dfs near-clique rmat k=5 forced stops", flush=True)
+    cfg = dm.dfs_config(vwave=32, threshold=100, stop_every_us=2000, max_rebalances=8, poll_us=500)
+    tot, _, st = dm.Dfs(g, 5, dm.DM_PATTERN_NEAR_CLIQUE, cfg).run()
+    print("  rebalances %d donated %d total %.1f ms" % (st["rebalances"], st["donated_jobs"], st["total_ms"]),
+          flush=True)
+    assert tot == want
 
 
 def test_kernel_of_another_width_counts_nothing(citeseer):

@@ -118,3 +118,21 @@ def test_learned_gain_over_plain_is_measured():
     print("spearman at k=4: model %.3f vs degree %.3f; k=5 kernel plain %.3f ms, learned %.3f ms (x%.2f)"
           % (rho_model, rho_degree, t["plain"], t["learned"], t["plain"] / t["learned"]), flush=True)
     assert rho_model > 0.3
+
+
+def test_learned_pieces_with_prebuilt_bitmaps_rmat():
+    # the learned partition cuts its highest-ranked roots into many pieces
+    # whose local bitmaps a pre-pass builds once; every count stays exact
+    import json
+    import os
+    from dumato_amd import workloads as W
+    gold = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "workload_counts.json")))["rmat_s17"]
+    g = W.rmat_s17()
+    print("case learned rmat train", flush=True)
+    g.sched_train(False, 4, epochs=30, seed=1)
+    for k in (4, 5, 6):
+        for cfg in (dm.config(sched=dm.DM_SCHED_LEARNED), dm.config(sched=dm.DM_SCHED_LEARNED, vwave=32, steal=0),
+                    dm.config(sched=dm.DM_SCHED_LEARNED, clique_bitmap=0)):
+            print("case learned rmat k=%d vwave %d steal %d bitmap %d" % (k, cfg.vwave, cfg.steal, cfg.clique_bitmap),
+                  flush=True)
+            assert g.clique_count(k, cfg) == gold["clique"][str(k)]

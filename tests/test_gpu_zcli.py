@@ -18,13 +18,41 @@ GOLD = os.path.join(HERE, "golden", "datasets")
 pytestmark = pytest.mark.gpu
 
 
-def _run(tool, gp, k, validate):
+def _run(tool, gp, k, validate, host_lb=False):
     env = dict(os.environ, DM_VALIDATE="1" if validate else "0")
-    print("case cli %s %s k=%d validate=%d" % (tool, os.path.basename(gp), k, validate), flush=True)
+    if host_lb:
+        env.update(DM_LB="host", DM_LB_POLL_US="200")
+    print("case cli %s %s k=%d validate=%d host_lb=%d" % (tool, os.path.basename(gp), k, validate, host_lb),
+          flush=True)
     r = subprocess.run([os.path.join(BIN, tool), gp, str(k), "102400", "256", "16", "30"],
                        capture_output=True, text=True, env=env, timeout=100)
     assert r.returncode == 0, r.stderr
     return r.stdout
+
+
+@pytest.mark.parametrize("k", [4, 5, 6])
+def test_q_pa_tools(citeseer, k):
+    # q4_PA .. q13_PA drop-ins: the near-clique query of size k
+    want = O.near_clique_count(O.load_edgelist(citeseer), k)
+    for tool in ("q%d_PA" % k, "q13_PA"):  # the size comes from argv, as in the reference
+        out = _run(tool, citeseer, k, False)
+        assert re.search(r"^Result: %d\.$" % want, out, re.M), out
+        assert re.search(r"^Time \(LB\): [0-9.]+ secs,[0-9.]+%$", out, re.M), out
+
+
+@pytest.mark.parametrize("k", [4, 5])
+def test_cli_host_driven_balancing(citeseer, k):
+    # DM_LB=host: the reference's model (step-API kernel, argv
+    # donationsPerWarp / threshold, host stop/snapshot/donate loop)
+    ref = O.load_edgelist(citeseer)
+    out = _run("clique_PA", citeseer, k, False, host_lb=True)
+    assert re.search(r"^Result: %d\.$" % O.clique_count(ref, k), out, re.M), out
+    assert re.search(r"^Time \(LB\): [0-9.]+ secs,[0-9.]+%$", out, re.M), out
+    tot, hist = O.motif_count(ref, k)
+    out = _run("motifs", citeseer, k, False, host_lb=True)
+    lines = dict((int(a), int(b)) for a, b in re.findall(r"^(\d+), (\d+)$", out, re.M))
+    assert lines == {i: int(v) for i, v in enumerate(hist) if v}, out
+    assert re.search(r"^Aggregated: %d, %d patterns\.$" % (tot, np.count_nonzero(hist)), out, re.M), out
 
 
 @pytest.fixture

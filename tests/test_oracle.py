@@ -228,3 +228,36 @@ def test_bytes_closed_forms(n):
     for k in (3, 4, 5):
         inner = 2 * m + sum(math.comb(m, j - 1) * (m + j - 1) for j in range(2, k))
         assert O.motif_bytes(gs, k) == 4 * (inner + (k - 1) * math.comb(m, k - 1))
+
+
+def _py_near_clique(src, dst, n, k):
+    """Pure-Python count of the near-clique query by k-subsets."""
+    adj = [set() for _ in range(n)]
+    for a, b in zip(src.tolist(), dst.tolist()):
+        if a != b:
+            adj[a].add(b)
+            adj[b].add(a)
+    total = 0
+    for S in itertools.combinations(range(n), k):
+        missing = sum(1 for a, b in itertools.combinations(S, 2) if b not in adj[a])
+        total += math.comb(k, 2) if missing == 0 else (1 if missing == 1 else 0)
+    return total
+
+
+@pytest.mark.parametrize("k", [4, 5, 6])
+def test_near_clique_query_pinned(k):
+    # q4_PA .. q13_PA: closed form on complete graphs, the C brute force and a
+    # pure-Python brute force on random graphs, faithful = fast
+    for n in (6, 9):
+        g = O.csr_from_edges(*G.complete(n)[:2], n)
+        assert O.near_clique_count(g, k) == math.comb(n, k) * math.comb(k, 2)
+    for seed in range(4):
+        s, d, n = G.gnp(13, 0.5, 40 + seed)
+        g = O.csr_from_edges(s, d, n)
+        want = _py_near_clique(s, d, n, k)
+        assert O.brute_near_clique(g, k) == want
+        assert O.near_clique_count(g, k) == want == O.near_clique_count(g, k, faithful=True)
+    # one missing edge in K_k: exactly one occurrence
+    s, d, n = G.complete(k)
+    keep = ~((s == 0) & (d == 1))
+    assert O.near_clique_count(O.csr_from_edges(s[keep], d[keep], n), k) == 1
