@@ -1219,19 +1219,30 @@ struct JobHead {
     bool donated;
 };
 
+// static queues: one per q_group workers (one per worker when q_group <= 1)
+DEV int num_queues(const EnumParams &p) {
+    const int gq = p.q_group > 1 ? p.q_group : 1;
+    return (p.nworkers + gq - 1) / gq;
+}
+DEV int own_queue(const EnumParams &p, int wid) { return p.q_group > 1 ? wid / p.q_group : wid; }
+
+// Claims the next job of static queue w.
 template <int W>
 DEV bool take_static(const EnumParams &p, const Grp<W> &g, int w, const WorkerLDS &ws, JobHead &jh) {
-    // strided partition (no job arrays): worker w holds roots n-1-w,
-    // n-1-w-nworkers, ...
+    // strided partition (no job arrays, one queue per worker): worker w
+    // holds roots n-1-w, n-1-w-nworkers, ...
     const bool strided = p.sq_jobs == nullptr;
     int64_t b = 0, e = 0;
     if (strided) e = w < p.n ? (int64_t)((p.n - 1 - w) / p.nworkers) + 1 : 0;
     else b = p.sq_off[w], e = p.sq_off[w + 1];
     int got = 0, r = 0, lo = 0, hi = 0, q = 0;
+    // a shared queue's cursor sits at its first worker's slot, so cursors of
+    // groups of 32 or more workers never share a cache line
+    int32_t *cursor = &p.sq_cur[p.q_group > 1 ? w * p.q_group : w];
     if (g.gl == 0 && b < e) {
-        int cur = __hip_atomic_load(&p.sq_cur[w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        int cur = __hip_atomic_load(cursor, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (b + cur < e) {
-            int j = atomicAdd(&p.sq_cur[w], 1);
+            int j = atomicAdd(cursor, 1);
             if (b + j < e) {
                 got = 1;
                 if (strided) {
@@ -1333,9 +1344,14 @@ DEV bool steal_static(const EnumParams &p, const Grp<W> &g, int w, const WorkerL
         if (g.gl == 0) ws.st()[kStStatic] = wall_clock64();
         return false;
     }
+    const int nq = num_queues(p);
+    if (nq <= 1) return false;  // the one queue was tried by take_static
+    // random probes: idle workers spread over the queues (a scan for the
+    // fullest queue sends them all to the same cursor, and device-scope
+    // atomics on one address slow every worker's memory traffic)
     for (int probe = 0; probe < 8; probe++) {
         seed = seed * 1664525u + 1013904223u;
-        int v = (int)((seed >> 8) % (uint32_t)p.nworkers);
+        int v = (int)((seed >> 8) % (uint32_t)nq);
         if (v == w) continue;
         if (take_static<W>(p, g, v, ws, jh)) return true;
     }
@@ -1410,8 +1426,8 @@ __global__ __launch_bounds__(kThreads, (min_waves<W, MODE>())) void enum_kernel(
             JobHead jh;
             bool got = false;
             if (!static_done) {
-                got = take_static<W>(p, g, wid, ws, jh);
-                if (!got) got = steal_static<W>(p, g, wid, ws, jh, seed, static_done);
+                got = take_static<W>(p, g, own_queue(p, wid), ws, jh);
+                if (!got) got = steal_static<W>(p, g, own_queue(p, wid), ws, jh, seed, static_done);
             }
             if (!got && static_done) got = poll_ticket<W>(p, g, ws, jh, has_ticket, ticket, wk.epoch);
             if (!got) {
@@ -1564,6 +1580,8 @@ __global__ __launch_bounds__(256) void prebuild_kernel(EnumParams p) {
     __shared__ uint8_t hval[512];
     __shared__ int64_t rst[256];
     __shared__ int32_t spre[257];
+    __shared__ int32_t wsum[4];
+    __shared__ unsigned long long srows[256 * 4];
     const int b = blockIdx.x, tid = threadIdx.x;
     if (b >= p.npre) return;
     const int32_t v = p.pre_roots[b];
@@ -1576,42 +1594,74 @@ __global__ __launch_bounds__(256) void prebuild_kernel(EnumParams p) {
     uint64_t *rows = (uint64_t *)const_cast<char *>(p.pre + p.pre_offs[b]);
     int32_t *gdp = (int32_t *)(rows + (int64_t)d * NW);
     for (int t = tid; t < HS; t += blockDim.x) hkey[t] = -1;
-    for (int i = tid; i < d; i += blockDim.x) {
-        const int32_t u = L[i];
+    // one child per thread (d <= 256 = blockDim.x): its list start, its
+    // out-degree and the block-wide exclusive prefix of the out-degrees
+    int od = 0;
+    if (tid < d) {
+        const int32_t u = L[tid];
         const int64_t os = p.ostart[u];
-        rst[i] = os;
-        gdp[i] = (int)(p.off[u + 1] - os);
+        rst[tid] = os;
+        od = (int)(p.off[u + 1] - os);
+        gdp[tid] = od;
     }
-    for (int t = tid; t < d * NW; t += blockDim.x) rows[t] = 0;
+    int incl = od;  // inclusive scan within the wave
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(incl, o, 64);
+        if ((tid & 63) >= o) incl += y;
+    }
+    if ((tid & 63) == 63) wsum[tid >> 6] = incl;
+    for (int t = tid; t < d * NW; t += blockDim.x) srows[t] = 0;
     __syncthreads();
+    int wbase = 0;
+    for (int w = 0; w < (tid >> 6); w++) wbase += wsum[w];
+    if (tid < d) spre[tid + 1] = wbase + incl;
+    if (tid == 0) spre[0] = 0;
     for (int j = tid; j < d; j += blockDim.x) {
         const int32_t key = L[j];
         uint32_t h = ((uint32_t)key * 2654435761u) >> (32 - HB);
         while (atomicCAS(&hkey[h], -1, key) != -1) h = (h + 1) & (HS - 1);
         hval[h] = (uint8_t)j;
     }
-    if (tid == 0) {
-        spre[0] = 0;
-        for (int i = 0; i < d; i++) spre[i + 1] = spre[i] + (int)(p.off[L[i] + 1] - rst[i]);
+    __syncthreads();
+    // flattened (row, neighbour) elements, U loads in flight per thread;
+    // bits go into the LDS copy of the rows
+    const int T = spre[d];
+    constexpr int U = 4;
+    for (int t0 = 0; t0 < T; t0 += U * 256) {
+        int32_t xs[U];
+        int ra[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int t = t0 + u * 256 + tid;
+            ra[u] = -1;
+            xs[u] = 0;
+            if (t < T) {
+                int a = 0, e = d - 1;  // the row of element t: spre[a] <= t < spre[a + 1]
+                while (a < e) {
+                    const int mid = (a + e + 1) >> 1;
+                    if (spre[mid] <= t) a = mid;
+                    else e = mid - 1;
+                }
+                ra[u] = a;
+                xs[u] = p.adj[rst[a] + (t - spre[a])];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            if (ra[u] < 0) continue;
+            const int32_t x = xs[u];
+            uint32_t h = ((uint32_t)x * 2654435761u) >> (32 - HB);
+            int k;
+            while ((k = hkey[h]) != x && k != -1) h = (h + 1) & (HS - 1);
+            if (k == x) {
+                const int j = hval[h];
+                atomicOr(&srows[ra[u] * NW + (j >> 6)], 1ull << (j & 63));
+            }
+        }
     }
     __syncthreads();
-    const int T = spre[d];
-    for (int t = tid; t < T; t += blockDim.x) {
-        int a = 0, e = d - 1;  // the row of element t: spre[a] <= t < spre[a + 1]
-        while (a < e) {
-            const int mid = (a + e + 1) >> 1;
-            if (spre[mid] <= t) a = mid;
-            else e = mid - 1;
-        }
-        const int32_t x = p.adj[rst[a] + (t - spre[a])];
-        uint32_t h = ((uint32_t)x * 2654435761u) >> (32 - HB);
-        int k;
-        while ((k = hkey[h]) != x && k != -1) h = (h + 1) & (HS - 1);
-        if (k == x) {
-            const int j = hval[h];
-            atomicOr(&rows[(int64_t)a * NW + (j >> 6)], 1ull << (j & 63));
-        }
-    }
+    for (int t = tid; t < d * NW; t += blockDim.x) rows[t] = srows[t];
 }
 
 int prebuild_launch(const EnumParams &p, void *stream) {

@@ -165,7 +165,10 @@ struct EnumParams {
     const int64_t *pre_offs;
     const char *pre;
     int32_t npre;
-    int32_t pad5;
+    // workers per static queue (0 or 1: one queue per worker): worker w
+    // claims from queue w / q_group of the (nworkers + q_group - 1) / q_group
+    // queues, and an idle worker probes the other queues
+    int32_t q_group;
 };
 
 // A static or donated job's query word: bits 0..15 the query index, bits

@@ -503,6 +503,7 @@ int run_enum(dm_graph *g, int K, bool motif, const dm_config &cfg, uint64_t *his
     p.pre_roots = npre ? d->d_pre_roots : nullptr;
     p.pre_offs = npre ? d->d_pre_offs : nullptr;
     p.pre = npre ? d->d_pre : nullptr;
+    p.q_group = part->group;
     int queues = 0;
     for (size_t w = 0; w + 1 < sq_off.size(); w++) queues += sq_off[w + 1] > sq_off[w] ? 1 : 0;
     if ((r = enum_init_launch(p, (unsigned long long)njobs, queues, d->d_epoch, d->d_results, nres, nullptr, 0, s)))

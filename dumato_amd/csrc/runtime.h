@@ -106,6 +106,7 @@ struct dm_graph {
         std::vector<int64_t> sq_off;
         std::vector<int32_t> sq_jobs;
         uint64_t skipped_bytes;
+        int group = 1;  // workers per static queue (sq_off has one entry per queue, + 1)
         // roots whose local bitmaps a pre-pass builds once for their pieces
         // (ascending), their offsets in the prebuilt buffer, its size
         std::vector<int32_t> pre_roots;

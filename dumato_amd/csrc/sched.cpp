@@ -12,7 +12,6 @@
 #include <cmath>
 #include <cstring>
 #include <numeric>
-#include <queue>
 
 #include "internal.h"
 #include "mlp.h"
@@ -389,18 +388,21 @@ static int predict_all(dm_graph *g, std::vector<float> &pred) {
 // Learned partition.  The model's predicted cost ranks the roots (it is
 // trained on the costs of a smaller k, so its ranking, not its magnitude, is
 // what carries over to the queried k):
-//  * the H = max(64, 2%) highest-ranked roots are cut into pieces of equal
+//  * the H = max(64, 5%) highest-ranked roots are cut into pieces of equal
 //    estimated work by first-level child ranges, 64 pieces for the top 32,
-//    32 for the next 96, 16 for the next 384, 8 for the rest, and each piece
+//    32 for the next 96, 16 for the next 384, 8 down to 2% of the roots and
+//    4 for the rest (the ranking of a larger k drifts from the trained
+//    one's, so roots heavy at k can sit lower in it), and each piece
 //    donates from as few as 2 children left; on the clique bitmap path
 //    (k = 4..6, out-degree <= 256) a pre-pass builds their local bitmaps
 //    once, so a piece copies its rows instead of rebuilding them;
 //  * the lower half of the roots keeps 16 children before it donates (a
 //    light job is not worth rebuilding its prefix elsewhere); the rest keep
 //    the launch's threshold.
-// The heaviest 2 x workers jobs go longest-processing-time first to the
-// least-loaded worker; the rest are dealt in snake order by predicted cost.
-// Everything but the head is bucketed, so the host cost stays linear.
+// All jobs go into one list, heaviest predicted first, that the workers claim
+// from in order (dynamic longest-first list scheduling).  The orderings are
+// counting sorts (exact only over the heavy head), so the host cost stays
+// linear in the number of roots.
 int learned_partition(dm_graph *g, const std::vector<int32_t> &roots, int K, bool motif, const dm_config &cfg,
                       const Shape &sh, dm_graph::Partition &part) {
     if (!g->model) return set_error(DM_ERR_ARG, "learned scheduler: train a model first (dm_sched_train)");
@@ -424,128 +426,143 @@ int learned_partition(dm_graph *g, const std::vector<int32_t> &roots, int K, boo
     };
     const size_t nr = roots.size();
     const size_t W = (size_t)std::max(sh.workers, 1);
-    const size_t H = std::min(nr, std::max<size_t>(64, nr / 50));
-    auto key = [&](int32_t i) { return pred[roots[i]]; };
+    const size_t H8 = std::min(nr, std::max<size_t>(64, nr / 50));
+    const size_t H = std::min(nr, std::max<size_t>(64, nr / 20));
+    // roots in descending predicted cost: a counting sort on 4096 buckets of
+    // the predicted log cost (ascending root within a bucket), made exact
+    // over the buckets that hold the H heaviest
+    std::vector<float> kv(nr);
+    float pmin = 1e30f, pmax = -1e30f;
+    for (size_t i = 0; i < nr; i++) {
+        kv[i] = pred[roots[i]];
+        pmin = std::min(pmin, kv[i]);
+        pmax = std::max(pmax, kv[i]);
+    }
+    constexpr int NB = 4096;
+    const float scale = pmax > pmin ? (float)(NB - 1) / (pmax - pmin) : 0.f;
+    std::vector<uint16_t> bk(nr);
+    std::vector<int32_t> bstart(NB + 1, 0);
+    for (size_t i = 0; i < nr; i++) {
+        bk[i] = (uint16_t)std::min(NB - 1, std::max(0, (int)((pmax - kv[i]) * scale)));
+        bstart[bk[i] + 1]++;
+    }
+    for (int b = 0; b < NB; b++) bstart[b + 1] += bstart[b];
     std::vector<int32_t> idx(nr);
-    std::iota(idx.begin(), idx.end(), 0);
-    auto desc = [&](int32_t a, int32_t b) { return key(a) > key(b) || (key(a) == key(b) && a < b); };
-    std::nth_element(idx.begin(), idx.begin() + H, idx.end(), desc);
-    std::sort(idx.begin(), idx.begin() + H, desc);
-    const bool halves = nr > 2 * H;
-    if (halves) std::nth_element(idx.begin() + H, idx.begin() + nr / 2, idx.end(), desc);
+    {
+        std::vector<int32_t> fill(bstart.begin(), bstart.end() - 1);
+        for (size_t i = 0; i < nr; i++) idx[fill[bk[i]]++] = (int32_t)i;
+    }
+    if (H > 0) {
+        const size_t hend = (size_t)bstart[bk[idx[H - 1]] + 1];
+        std::sort(idx.begin(), idx.begin() + hend, [&](int32_t a, int32_t b) {
+            return kv[a] > kv[b] || (kv[a] == kv[b] && a < b);
+        });
+    }
 
     const bool can_split = !(K == 2 && !motif);
     const bool bitmap_path = !motif && cfg.clique_bitmap && K >= 4 && K <= 6;
     const int e = motif ? 1 : std::max(1, K - 3);
-    auto wt = [e](int m) {  // m^e
+    // child i of a root with nch children carries about (nch - i)^e of its
+    // work (later children see fewer candidates above them); cw[m] = sum of
+    // x^e for x = 1..m, so children [lo, hi) weigh cw[nch - lo] - cw[nch - hi]
+    int maxch = 0;
+    for (size_t rk = 0; rk < H; rk++) {
+        const int32_t v = roots[idx[rk]];
+        maxch = std::max(maxch, (int)(g->rcsr.off[v + 1] - g->ostart[v]));
+    }
+    std::vector<double> cw((size_t)maxch + 1, 0.0);
+    for (int m = 1; m <= maxch; m++) {
         double x = 1.0;
         for (int i = 0; i < e; i++) x *= (double)m;
-        return x;
-    };
+        cw[m] = cw[m - 1] + x;
+    }
     std::vector<Job> jobs;
     jobs.reserve(nr + H * 16);
     part.pre_roots.clear();
     for (size_t rk = 0; rk < H; rk++) {
         const int32_t v = roots[idx[rk]];
-        const double c = std::exp((double)key(idx[rk]));
+        const double c = std::exp((double)kv[idx[rk]]);
         const int nch = (int)(g->rcsr.off[v + 1] - g->ostart[v]);  // children of the root
-        int P = rk < 32 ? 64 : rk < 128 ? 32 : rk < 512 ? 16 : 8;
+        int P = rk < 32 ? 64 : rk < 128 ? 32 : rk < 512 ? 16 : rk < H8 ? 8 : 4;
         P = can_split ? std::min(P, nch) : 1;
         const int32_t q = make_job_query(0, 2);
         if (P <= 1) {
             jobs.push_back({v, 0, INT_MAX, q, c});
             continue;
         }
-        // child i carries about (nch - i)^e of the root's work (later
-        // children see fewer candidates above them)
-        double ws = 0;
-        for (int i = 0; i < nch; i++) ws += wt(nch - i);
-        double acc = 0;
+        const double ws = cw[nch];
         int lo = 0;
         for (int pi = 1; pi <= P && lo < nch; pi++) {
-            const double goal = ws * pi / P;
-            int hi = lo;
-            double a = acc;
-            while (hi < nch && (a < goal || hi == lo)) a += wt(nch - hi++);
-            if (pi == P)
-                while (hi < nch) a += wt(nch - hi++);
-            jobs.push_back({v, lo, hi >= nch ? INT_MAX : hi, q, c * (a - acc) / ws});
-            acc = a;
+            // the first hi > lo whose prefix [0, hi) reaches pi/P of the work
+            int hi = nch;
+            if (pi < P) {
+                const double goal = ws * pi / P;
+                int a = lo + 1, b = nch;
+                while (a < b) {
+                    const int mid = (a + b) >> 1;
+                    if (ws - cw[nch - mid] >= goal) b = mid;
+                    else a = mid + 1;
+                }
+                hi = a;
+            }
+            jobs.push_back({v, lo, hi >= nch ? INT_MAX : hi, q, c * (cw[nch - lo] - cw[nch - hi]) / ws});
             lo = hi;
         }
         if (bitmap_path && nch <= 256) part.pre_roots.push_back(v);
     }
     const size_t nheavy = jobs.size();
-    // the other roots in descending predicted cost: 256 buckets of the
-    // predicted log cost
-    float pmin = 1e30f, pmax = -1e30f;
+    // the rest in bucket order; the lower half (by position) is light
     for (size_t t = H; t < nr; t++) {
-        pmin = std::min(pmin, key(idx[t]));
-        pmax = std::max(pmax, key(idx[t]));
-    }
-    const float scale = pmax > pmin ? 255.f / (pmax - pmin) : 0.f;
-    std::vector<int32_t> bstart(257, 0);
-    auto bucket = [&](int32_t i) { return std::min(255, std::max(0, (int)((pmax - key(i)) * scale))); };
-    for (size_t t = H; t < nr; t++) bstart[bucket(idx[t]) + 1]++;
-    for (int b = 0; b < 256; b++) bstart[b + 1] += bstart[b];
-    std::vector<int32_t> tail(nr - H);
-    for (size_t t = H; t < nr; t++) tail[bstart[bucket(idx[t])]++] = idx[t];
-    std::vector<char> light(nr, 0);
-    if (halves)
-        for (size_t t = nr / 2; t < nr; t++) light[idx[t]] = 1;
-    for (int32_t i : tail) {
-        const int32_t q = light[i] ? make_job_query(0, 16) : 0;
+        const int32_t i = idx[t];
+        const int32_t q = nr > 2 * H && t >= nr / 2 ? make_job_query(0, 16) : 0;
         jobs.push_back({roots[i], 0, INT_MAX, q, 0.0});
     }
 
-    // deal: LPT over the heaviest jobs (the pieces and the highest-ranked
-    // roots, already in rank order), then snake over the least-loaded order
+    // one list, heaviest first: the pieces and heavy roots by predicted cost
+    // (a counting sort of its log), then the rest in rank order.  It is dealt
+    // round-robin to queues shared by kQueueGroup workers each, which claim
+    // in list order: longest-first list scheduling inside a group, with a
+    // group's cursor shared by few enough workers that its atomics do not
+    // contend (one list for all workers ran 2.6x slower: device-scope
+    // atomics on one address slowed every worker's memory traffic).
+    constexpr size_t kQueueGroup = 8;
     const size_t J = jobs.size();
-    std::vector<int32_t> owner(J);
-    const size_t head = std::min(nheavy, 2 * W);
-    std::vector<size_t> hord(nheavy);
-    std::iota(hord.begin(), hord.end(), 0);
-    std::partial_sort(hord.begin(), hord.begin() + head, hord.end(),
-                      [&](size_t a, size_t b) { return jobs[a].cost > jobs[b].cost; });
-    using QE = std::pair<double, int32_t>;
-    std::priority_queue<QE, std::vector<QE>, std::greater<QE>> heap;
-    for (size_t w = 0; w < W; w++) heap.push({0.0, (int32_t)w});
-    for (size_t t = 0; t < head; t++) {
-        QE x = heap.top();
-        heap.pop();
-        owner[hord[t]] = x.second;
-        x.first += jobs[hord[t]].cost;
-        heap.push(x);
+    std::vector<int32_t> hord(nheavy);
+    {
+        float cmin = 1e30f, cmax = -1e30f;
+        std::vector<float> lc(nheavy);
+        for (size_t t = 0; t < nheavy; t++) {
+            lc[t] = (float)std::log(std::max(jobs[t].cost, 1e-300));
+            cmin = std::min(cmin, lc[t]);
+            cmax = std::max(cmax, lc[t]);
+        }
+        const float cs = cmax > cmin ? (float)(NB - 1) / (cmax - cmin) : 0.f;
+        std::vector<int32_t> cb(NB + 1, 0);
+        std::vector<uint16_t> hb(nheavy);
+        for (size_t t = 0; t < nheavy; t++) {
+            hb[t] = (uint16_t)std::min(NB - 1, std::max(0, (int)((cmax - lc[t]) * cs)));
+            cb[hb[t] + 1]++;
+        }
+        for (int b = 0; b < NB; b++) cb[b + 1] += cb[b];
+        for (size_t t = 0; t < nheavy; t++) hord[cb[hb[t]]++] = (int32_t)t;
     }
-    std::vector<int32_t> by_load(W);
-    for (size_t w = 0; w < W; w++) {
-        by_load[w] = heap.top().second;  // least loaded first
-        heap.pop();
-    }
-    size_t s = 0;
-    auto deal = [&](size_t j) {
-        const size_t lap = s / W, pos = s % W;
-        owner[j] = by_load[(lap & 1) ? W - 1 - pos : pos];
-        s++;
-    };
-    for (size_t t = head; t < nheavy; t++) deal(hord[t]);
-    for (size_t j = nheavy; j < J; j++) deal(j);
-
-    // queues, in dealing order within each worker (LPT head first)
-    part.sq_off.assign(W + 1, 0);
-    for (size_t j = 0; j < J; j++) part.sq_off[(size_t)owner[j] + 1]++;
-    for (size_t w = 0; w < W; w++) part.sq_off[w + 1] += part.sq_off[w];
-    std::vector<int64_t> fill(part.sq_off.begin(), part.sq_off.end() - 1);
-    part.sq_jobs.assign(J * 4, 0);
-    auto put = [&](size_t j) {
-        int32_t *jp = &part.sq_jobs[4 * (size_t)fill[owner[j]]++];
+    const size_t G = std::min(W, kQueueGroup);
+    const size_t nq = (W + G - 1) / G;
+    part.group = (int)G;
+    // position t of the list goes to queue t % nq, in list order
+    part.sq_off.assign(nq + 1, 0);
+    for (size_t qi = 0; qi < nq; qi++) part.sq_off[qi + 1] = part.sq_off[qi] + (int64_t)((J + nq - 1 - qi) / nq);
+    part.sq_jobs.resize(J * 4);
+    auto put = [&](size_t t, size_t j) {
+        const size_t qi = t % nq;
+        int32_t *jp = &part.sq_jobs[4 * (size_t)(part.sq_off[qi] + (int64_t)(t / nq))];
         jp[0] = jobs[j].root;
         jp[1] = jobs[j].lo;
         jp[2] = jobs[j].hi;
         jp[3] = jobs[j].q;
     };
-    for (size_t t = 0; t < nheavy; t++) put(hord[t]);
-    for (size_t j = nheavy; j < J; j++) put(j);
+    for (size_t t = 0; t < nheavy; t++) put(t, (size_t)hord[t]);
+    for (size_t j = nheavy; j < J; j++) put(j, j);
 
     // prebuilt bitmaps, ascending by root
     std::sort(part.pre_roots.begin(), part.pre_roots.end());
