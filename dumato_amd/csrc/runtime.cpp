@@ -405,6 +405,16 @@ int run_enum(dm_graph *g, int K, bool motif, const dm_config &cfg, uint64_t *his
             else if (motif) skipped_bytes += 4ull * (uint64_t)(g->rcsr.off[v + 1] - g->rcsr.off[v]);
         }
         dm_graph::Partition np;
+        // a partition recomputed for the same key takes over the old one's
+        // arrays: their capacity spares the new ones first-touch page faults
+        for (dm_graph::Partition &c : g->partitions)
+            if (std::equal(key, key + 7, c.key)) {
+                np.sq_off.swap(c.sq_off);
+                np.sq_jobs.swap(c.sq_jobs);
+                np.pre_roots.swap(c.pre_roots);
+                np.pre_offs.swap(c.pre_offs);
+                break;
+            }
         std::copy(key, key + 7, np.key);
         np.model_gen = g->model_gen;
         np.skipped_bytes = skipped_bytes;

@@ -422,7 +422,7 @@ int learned_partition(dm_graph *g, const std::vector<int32_t> &roots, int K, boo
 
     struct Job {
         int32_t root, lo, hi, q;
-        double cost;
+        float lcost;  // predicted log cycles
     };
     const size_t nr = roots.size();
     const size_t W = (size_t)std::max(sh.workers, 1);
@@ -431,7 +431,13 @@ int learned_partition(dm_graph *g, const std::vector<int32_t> &roots, int K, boo
     // roots in descending predicted cost: a counting sort on 4096 buckets of
     // the predicted log cost (ascending root within a bucket), made exact
     // over the buckets that hold the H heaviest
-    std::vector<float> kv(nr);
+    // per-root and per-job scratch keeps its capacity between calls (a fresh
+    // allocation's first-touch page faults cost as much as the binning)
+    static thread_local std::vector<float> kv;
+    static thread_local std::vector<uint16_t> bk;
+    static thread_local std::vector<int32_t> idx, hord;
+    static thread_local std::vector<uint16_t> hb;
+    kv.resize(nr);
     float pmin = 1e30f, pmax = -1e30f;
     for (size_t i = 0; i < nr; i++) {
         kv[i] = pred[roots[i]];
@@ -440,14 +446,14 @@ int learned_partition(dm_graph *g, const std::vector<int32_t> &roots, int K, boo
     }
     constexpr int NB = 4096;
     const float scale = pmax > pmin ? (float)(NB - 1) / (pmax - pmin) : 0.f;
-    std::vector<uint16_t> bk(nr);
+    bk.resize(nr);
     std::vector<int32_t> bstart(NB + 1, 0);
     for (size_t i = 0; i < nr; i++) {
         bk[i] = (uint16_t)std::min(NB - 1, std::max(0, (int)((pmax - kv[i]) * scale)));
         bstart[bk[i] + 1]++;
     }
     for (int b = 0; b < NB; b++) bstart[b + 1] += bstart[b];
-    std::vector<int32_t> idx(nr);
+    idx.resize(nr);
     {
         std::vector<int32_t> fill(bstart.begin(), bstart.end() - 1);
         for (size_t i = 0; i < nr; i++) idx[fill[bk[i]]++] = (int32_t)i;
@@ -476,20 +482,22 @@ int learned_partition(dm_graph *g, const std::vector<int32_t> &roots, int K, boo
         for (int i = 0; i < e; i++) x *= (double)m;
         cw[m] = cw[m - 1] + x;
     }
-    std::vector<Job> jobs;
-    jobs.reserve(nr + H * 16);
+    // heavy roots and their pieces; a piece's predicted log cost is its
+    // root's less log P (the cuts give the P pieces equal estimated work)
+    static thread_local std::vector<Job> jobs;
+    jobs.clear();
     part.pre_roots.clear();
     for (size_t rk = 0; rk < H; rk++) {
         const int32_t v = roots[idx[rk]];
-        const double c = std::exp((double)kv[idx[rk]]);
         const int nch = (int)(g->rcsr.off[v + 1] - g->ostart[v]);  // children of the root
         int P = rk < 32 ? 64 : rk < 128 ? 32 : rk < 512 ? 16 : rk < H8 ? 8 : 4;
         P = can_split ? std::min(P, nch) : 1;
         const int32_t q = make_job_query(0, 2);
         if (P <= 1) {
-            jobs.push_back({v, 0, INT_MAX, q, c});
+            jobs.push_back({v, 0, INT_MAX, q, kv[idx[rk]]});
             continue;
         }
+        const float lc = kv[idx[rk]] - (float)std::log((double)P);
         const double ws = cw[nch];
         int lo = 0;
         for (int pi = 1; pi <= P && lo < nch; pi++) {
@@ -505,42 +513,35 @@ int learned_partition(dm_graph *g, const std::vector<int32_t> &roots, int K, boo
                 }
                 hi = a;
             }
-            jobs.push_back({v, lo, hi >= nch ? INT_MAX : hi, q, c * (cw[nch - lo] - cw[nch - hi]) / ws});
+            jobs.push_back({v, lo, hi >= nch ? INT_MAX : hi, q, lc});
             lo = hi;
         }
         if (bitmap_path && nch <= 256) part.pre_roots.push_back(v);
     }
     const size_t nheavy = jobs.size();
-    // the rest in bucket order; the lower half (by position) is light
-    for (size_t t = H; t < nr; t++) {
-        const int32_t i = idx[t];
-        const int32_t q = nr > 2 * H && t >= nr / 2 ? make_job_query(0, 16) : 0;
-        jobs.push_back({roots[i], 0, INT_MAX, q, 0.0});
-    }
 
     // one list, heaviest first: the pieces and heavy roots by predicted cost
-    // (a counting sort of its log), then the rest in rank order.  It is dealt
-    // round-robin to queues shared by kQueueGroup workers each, which claim
-    // in list order: longest-first list scheduling inside a group, with a
-    // group's cursor shared by few enough workers that its atomics do not
-    // contend (one list for all workers ran 2.6x slower: device-scope
-    // atomics on one address slowed every worker's memory traffic).
+    // (a counting sort of its log), then the other roots in rank order, the
+    // lower half of them light.  It is dealt round-robin to queues shared by
+    // kQueueGroup workers each, which claim in list order: longest-first list
+    // scheduling inside a group, with a group's cursor shared by few enough
+    // workers that its atomics do not contend (one list for all workers ran
+    // 2.6x slower: device-scope atomics on one address slowed every worker's
+    // memory traffic).
     constexpr size_t kQueueGroup = 8;
-    const size_t J = jobs.size();
-    std::vector<int32_t> hord(nheavy);
+    const size_t J = nheavy + (nr - H);
+    hord.resize(nheavy);
     {
         float cmin = 1e30f, cmax = -1e30f;
-        std::vector<float> lc(nheavy);
         for (size_t t = 0; t < nheavy; t++) {
-            lc[t] = (float)std::log(std::max(jobs[t].cost, 1e-300));
-            cmin = std::min(cmin, lc[t]);
-            cmax = std::max(cmax, lc[t]);
+            cmin = std::min(cmin, jobs[t].lcost);
+            cmax = std::max(cmax, jobs[t].lcost);
         }
         const float cs = cmax > cmin ? (float)(NB - 1) / (cmax - cmin) : 0.f;
         std::vector<int32_t> cb(NB + 1, 0);
-        std::vector<uint16_t> hb(nheavy);
+        hb.resize(nheavy);
         for (size_t t = 0; t < nheavy; t++) {
-            hb[t] = (uint16_t)std::min(NB - 1, std::max(0, (int)((cmax - lc[t]) * cs)));
+            hb[t] = (uint16_t)std::min(NB - 1, std::max(0, (int)((cmax - jobs[t].lcost) * cs)));
             cb[hb[t] + 1]++;
         }
         for (int b = 0; b < NB; b++) cb[b + 1] += cb[b];
@@ -549,20 +550,24 @@ int learned_partition(dm_graph *g, const std::vector<int32_t> &roots, int K, boo
     const size_t G = std::min(W, kQueueGroup);
     const size_t nq = (W + G - 1) / G;
     part.group = (int)G;
-    // position t of the list goes to queue t % nq, in list order
+    // list position t goes to queue t % nq, in list order
     part.sq_off.assign(nq + 1, 0);
     for (size_t qi = 0; qi < nq; qi++) part.sq_off[qi + 1] = part.sq_off[qi] + (int64_t)((J + nq - 1 - qi) / nq);
     part.sq_jobs.resize(J * 4);
-    auto put = [&](size_t t, size_t j) {
-        const size_t qi = t % nq;
-        int32_t *jp = &part.sq_jobs[4 * (size_t)(part.sq_off[qi] + (int64_t)(t / nq))];
-        jp[0] = jobs[j].root;
-        jp[1] = jobs[j].lo;
-        jp[2] = jobs[j].hi;
-        jp[3] = jobs[j].q;
+    auto put = [&](size_t t, int32_t root, int32_t lo, int32_t hi, int32_t q) {
+        int32_t *jp = &part.sq_jobs[4 * (size_t)(part.sq_off[t % nq] + (int64_t)(t / nq))];
+        jp[0] = root;
+        jp[1] = lo;
+        jp[2] = hi;
+        jp[3] = q;
     };
-    for (size_t t = 0; t < nheavy; t++) put(t, (size_t)hord[t]);
-    for (size_t j = nheavy; j < J; j++) put(j, j);
+    for (size_t t = 0; t < nheavy; t++) {
+        const Job &jb = jobs[(size_t)hord[t]];
+        put(t, jb.root, jb.lo, jb.hi, jb.q);
+    }
+    const bool halves = nr > 2 * H;
+    for (size_t t = H; t < nr; t++)
+        put(nheavy + (t - H), roots[idx[t]], 0, INT_MAX, halves && t >= nr / 2 ? make_job_query(0, 16) : 0);
 
     // prebuilt bitmaps, ascending by root
     std::sort(part.pre_roots.begin(), part.pre_roots.end());

@@ -45,6 +45,12 @@
  *   or_clique_bytes / or_motif_bytes
  *       The algorithmic-byte formulas of the measurement plan; pinned by
  *       closed forms on complete graphs and stars.
+ *
+ * The counting loops hand out one root at a time to the OpenMP threads
+ * (schedule(dynamic, 1)), as the reference DFS baselines hand one root to
+ * each thread that asks (the atomicAdd root counter of dm_popJob,
+ * src/clique_DM_DFS/DuMato.h:66-88): with larger chunks one thread can hold
+ * several hubs while the others idle.
  */
 #include "oracle.h"
 
@@ -289,7 +295,7 @@ uint64_t or_clique_count_roots(int32_t n, const int64_t *off,
     (void)n;
     set_threads(nthreads);
     uint64_t total = 0;
-#pragma omp parallel for schedule(dynamic, 16) reduction(+ : total)
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : total)
     for (int64_t i = 0; i < nroots; i++) {
         uint64_t c = clique_root(off, adj, k, faithful, roots[i]);
         if (per_root_out) per_root_out[i] = c;
@@ -304,7 +310,7 @@ uint64_t or_clique_count(int32_t n, const int64_t *off, const int32_t *adj,
     if (k < 1 || k > OR_MAXK) return UINT64_MAX;
     set_threads(nthreads);
     uint64_t total = 0;
-#pragma omp parallel for schedule(dynamic, 16) reduction(+ : total)
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : total)
     for (int32_t v = 0; v < n; v++) {
         uint64_t c = clique_root(off, adj, k, faithful, v);
         if (per_root) per_root[v] = c;
@@ -370,7 +376,7 @@ uint64_t or_near_clique_count(int32_t n, const int64_t *off,
     if (k < 4 || k > OR_MAXK) return UINT64_MAX;
     set_threads(nthreads);
     uint64_t total = 0;
-#pragma omp parallel for schedule(dynamic, 16) reduction(+ : total)
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : total)
     for (int32_t v = 0; v < n; v++)
         total += near_clique_root(off, adj, k, faithful, v);
     return total;
@@ -557,7 +563,7 @@ static uint64_t motif_run(int32_t n, const int64_t *off, const int32_t *adj,
     {
         mctx_t c;
         motif_ctx_init(&c, n, off, adj, k, lut, ncls, faithful);
-#pragma omp for schedule(dynamic, 16)
+#pragma omp for schedule(dynamic, 1)
         for (int64_t i = 0; i < cnt; i++) {
             int32_t v = roots ? roots[i] : (int32_t)i;
             uint64_t t = motif_root(&c, v);
