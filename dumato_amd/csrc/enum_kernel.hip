@@ -913,7 +913,7 @@ struct Worker {
     }
 
     template <int NW>
-    DEV void clique_bitmap_root(int32_t v, int lo, int hi, bool first_piece) {
+    DEV void clique_bitmap_root(int32_t v, int lo, int hi, bool first_piece, bool shared) {
         const int d = odeg(v);
         const int32_t *L = onbr(v);
         Lvl &R = ws.lv()[0];
@@ -922,6 +922,11 @@ struct Worker {
         R.vtx = v;
         R.mark_lds = lds_top;
         R.mark_glb = g_top;
+        const int jobs_on_root = hi;  // shared: the root's job count
+        if (shared) {
+            lo = 0;
+            hi = d;
+        }
         if (hi > d) hi = d;
         if (lo >= hi) return;
         // pair-phase data first, so that it gets the LDS
@@ -947,12 +952,42 @@ struct Worker {
             for (int t = lo * NW + g.gl; t < d * NW; t += W) rows[t] = gr[t];
             for (int i = g.gl; i < d; i += W) dp[i] = gd[i];
             wsync();
+        } else if (shared) {
+            fail(3);  // the host gives a shared cursor only to prebuilt roots
+            return;
         } else {
             build_rows<NW>(v, lo, d, L, rows, dp, pre);
             if (abort) return;
         }
-        if (first_piece && g.gl == 0) bytes += 4ull * (unsigned long long)d;
-        pair_phase<NW>(R, rows, dp, pre, lo, hi);
+        if (shared) {
+            // guided self-scheduling with the root's other jobs: each claim
+            // takes about 1 / (2 x jobs on the root) of the children left, at
+            // least one, so the root's last children are spread thinly
+            // whatever their weights; the claim that starts at child 0 counts
+            // the root's bytes.  (Claims sized by estimated work, pop^(k-3)
+            // per child with a compare-and-swap cursor, balanced better but
+            // their extra claims cost more: R-MAT k = 6 4.78 -> 5.0 ms.)
+            int32_t *cursor = p.pre_cursor + slot;
+            const int share = 2 * (jobs_on_root > 1 ? jobs_on_root : 1);
+            while (!abort) {
+                int a = 0, n = 1;
+                if (g.gl == 0) {
+                    const int cur = __hip_atomic_load(cursor, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    n = d > cur ? (d - cur + share - 1) / share : 1;
+                    a = atomicAdd(cursor, n);
+                }
+                a = g.bcast(a, 0);
+                n = g.bcast(n, 0);
+                if (a >= d) break;
+                if (a == 0 && g.gl == 0) bytes += 4ull * (unsigned long long)d;
+                pair_phase<NW>(R, rows, dp, pre, a, a + n < d ? a + n : d);
+            }
+        } else {
+            if (first_piece && g.gl == 0) bytes += 4ull * (unsigned long long)d;
+            pair_phase<NW>(R, rows, dp, pre, lo, hi);
+        }
+        lds_top = R.mark_lds;
+        g_top = R.mark_glb;
     }
 
     // Rows lo..d-1 of the local bitmap of root v (L = N+(v), d = |L|): row i
@@ -1099,15 +1134,15 @@ struct Worker {
             wsync();
         }
         bytes += 4ull * by;
-        lds_top = R.mark_lds;
-        g_top = R.mark_glb;
     }
 
     // ---- one job ------------------------------------------------------------
     // children [lo, hi) of the node whose prefix ws.jid()[0..lvl] is in LDS;
     // first_piece: a static job that starts at the root's first child (the
-    // root node's own bytes are counted once, by that job)
-    DEV void run_job(int lvl, int lo, int hi, bool first_piece) {
+    // root node's own bytes are counted once, by that job); shared: one of
+    // `hi` jobs on a prebuilt bitmap root that claim its children from the
+    // root's shared cursor
+    DEV void run_job(int lvl, int lo, int hi, bool first_piece, bool shared) {
         v0 = ws.jid()[0];
         job_lvl = lvl;
         lds_top = lds_lo;
@@ -1129,17 +1164,21 @@ struct Worker {
         if (!motif && p.bitmap && lvl == 0 && K >= 4 && K <= 6) {
             const int d = odeg(v0);
             if (d <= 64) {
-                clique_bitmap_root<1>(v0, lo, hi, first_piece);
+                clique_bitmap_root<1>(v0, lo, hi, first_piece, shared);
                 return;
             }
             if (d <= 128) {
-                clique_bitmap_root<2>(v0, lo, hi, first_piece);
+                clique_bitmap_root<2>(v0, lo, hi, first_piece, shared);
                 return;
             }
             if (d <= 256) {
-                clique_bitmap_root<4>(v0, lo, hi, first_piece);
+                clique_bitmap_root<4>(v0, lo, hi, first_piece, shared);
                 return;
             }
+        }
+        if (shared) {  // only prebuilt bitmap roots have a shared cursor
+            fail(3);
+            return;
         }
         // rebuild the prefix levels 0..lvl
         if (motif) motif_root_level(v0);
@@ -1217,6 +1256,7 @@ struct JobHead {
     int lvl, lo, hi, q;
     int range_end;  // > 0: a static range job over roots [jid[0], range_end)
     bool donated;
+    bool shared;    // one of hi jobs that claim the root's children from its shared cursor
 };
 
 // static queues: one per q_group workers (one per worker when q_group <= 1)
@@ -1266,11 +1306,17 @@ DEV bool take_static(const EnumParams &p, const Grp<W> &g, int w, const WorkerLD
     if (!got) return false;
     jh.lvl = 0;
     jh.donated = false;
+    jh.shared = false;
     lo = g.bcast(lo, 0);
     hi = g.bcast(hi, 0);
     jh.q = g.bcast(q, 0);
     r = g.bcast(r, 0);
-    if (lo < 0) {  // range job
+    if (lo == -2) {  // (root, -2, jobs on the root, q): children from the shared cursor
+        jh.range_end = 0;
+        jh.shared = true;
+        jh.lo = 0;
+        jh.hi = hi;
+    } else if (lo < 0) {  // range job
         jh.range_end = hi;
         jh.lo = 0;
         jh.hi = 0x7fffffff;
@@ -1324,6 +1370,7 @@ DEV bool poll_ticket(const EnumParams &p, const Grp<W> &g, const WorkerLDS &ws, 
     jh.q = g.bcast(v[0], 3);
     jh.range_end = 0;
     jh.donated = true;
+    jh.shared = false;
     if (jh.lo < 0) {  // the rest of a range job: roots [jid[0], hi)
         jh.range_end = jh.hi;
         jh.lo = 0;
@@ -1502,7 +1549,9 @@ __global__ __launch_bounds__(kThreads, (min_waves<W, MODE>())) void enum_kernel(
                         if (!ws.st()[kStFirst]) ws.st()[kStFirst] = wall_clock64();
                         ws.st()[kStT0] = __builtin_amdgcn_s_memtime();
                     }
-                    wk.run_job(jh.lvl, jh.lo, jh.hi, jh.range_end > 0 || (!jh.donated && jh.lvl == 0 && jh.lo == 0));
+                    wk.run_job(jh.lvl, jh.lo, jh.hi,
+                               !jh.shared && (jh.range_end > 0 || (!jh.donated && jh.lvl == 0 && jh.lo == 0)),
+                               jh.shared);
                     unsigned long long jc = g.lsum(wk.cnt);
                     if (g.gl == 0) {
                         const unsigned long long dt = __builtin_amdgcn_s_memtime() - ws.st()[kStT0];

@@ -164,6 +164,10 @@ struct EnumParams {
     const int32_t *pre_roots;
     const int64_t *pre_offs;
     const char *pre;
+    // per prebuilt root: the next child its shared-cursor jobs claim (zeroed
+    // before each launch); a static job (root, -2, P, q) is one of the P
+    // jobs that share root's cursor
+    int32_t *pre_cursor;
     int32_t npre;
     // workers per static queue (0 or 1: one queue per worker): worker w
     // claims from queue w / q_group of the (nworkers + q_group - 1) / q_group

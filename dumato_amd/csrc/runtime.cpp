@@ -383,11 +383,13 @@ int run_enum(dm_graph *g, int K, bool motif, const dm_config &cfg, uint64_t *his
     g->last_infer_ms = g->last_bin_ms = 0;
     int32_t rb = std::max(cfg.root_begin, 0), re = cfg.root_end > 0 ? std::min(cfg.root_end, n) : n;
     const int learned = cfg.sched == DM_SCHED_LEARNED ? 1 : 0;
-    const int key[7] = {K, motif ? 1 : 0, learned, sh.workers, rb, re, sample_mod};
+    // the bitmap switch is part of the key: a learned partition gives heavy
+    // bitmap roots shared-cursor jobs, which need their prebuilt bitmaps
+    const int key[8] = {K, motif ? 1 : 0, learned, sh.workers, rb, re, sample_mod, cfg.clique_bitmap ? 1 : 0};
     dm_graph::Partition *part = nullptr;
     for (size_t i = 0; i < g->partitions.size() && !cfg.repartition; i++) {
         dm_graph::Partition &c = g->partitions[i];
-        if (std::equal(key, key + 7, c.key) && (!learned || c.model_gen == g->model_gen)) {
+        if (std::equal(key, key + 8, c.key) && (!learned || c.model_gen == g->model_gen)) {
             if (i) std::swap(g->partitions[0], c);
             part = &g->partitions[0];
             break;
@@ -408,14 +410,14 @@ int run_enum(dm_graph *g, int K, bool motif, const dm_config &cfg, uint64_t *his
         // a partition recomputed for the same key takes over the old one's
         // arrays: their capacity spares the new ones first-touch page faults
         for (dm_graph::Partition &c : g->partitions)
-            if (std::equal(key, key + 7, c.key)) {
+            if (std::equal(key, key + 8, c.key)) {
                 np.sq_off.swap(c.sq_off);
                 np.sq_jobs.swap(c.sq_jobs);
                 np.pre_roots.swap(c.pre_roots);
                 np.pre_offs.swap(c.pre_offs);
                 break;
             }
-        std::copy(key, key + 7, np.key);
+        std::copy(key, key + 8, np.key);
         np.model_gen = g->model_gen;
         np.skipped_bytes = skipped_bytes;
         if (learned) {
@@ -425,7 +427,7 @@ int run_enum(dm_graph *g, int K, bool motif, const dm_config &cfg, uint64_t *his
         }
         g->partitions.erase(std::remove_if(g->partitions.begin(), g->partitions.end(),
                                            [&](const dm_graph::Partition &c) {
-                                               return std::equal(key, key + 7, c.key);
+                                               return std::equal(key, key + 8, c.key);
                                            }),
                             g->partitions.end());
         if (g->partitions.size() >= 8) g->partitions.pop_back();
@@ -464,6 +466,7 @@ int run_enum(dm_graph *g, int K, bool motif, const dm_config &cfg, uint64_t *his
     if (npre > 0) {
         if ((r = dev_grow(d->d_pre_roots, d->pre_roots_cap, (size_t)npre)) ||
             (r = dev_grow(d->d_pre_offs, d->pre_offs_cap, (size_t)npre)) ||
+            (r = dev_grow(d->d_pre_cursor, d->pre_cursor_cap, (size_t)npre)) ||
             (r = dev_grow(d->d_pre, d->pre_cap, (size_t)part->pre_bytes)))
             return r;
     }
@@ -472,6 +475,7 @@ int run_enum(dm_graph *g, int K, bool motif, const dm_config &cfg, uint64_t *his
     if (npre > 0) {
         HIPCHK(hipMemcpyAsync(d->d_pre_roots, part->pre_roots.data(), (size_t)npre * 4, hipMemcpyHostToDevice, s));
         HIPCHK(hipMemcpyAsync(d->d_pre_offs, part->pre_offs.data(), (size_t)npre * 8, hipMemcpyHostToDevice, s));
+        if ((r = fill_u32_launch(d->d_pre_cursor, 0, (size_t)npre, s))) return r;
     }
     HIPCHK(hipMemcpyAsync(d->d_qd, &qd, sizeof(qd), hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync(d->d_sq_off, sq_off.data(), sq_off.size() * 8, hipMemcpyHostToDevice, s));
@@ -513,6 +517,7 @@ int run_enum(dm_graph *g, int K, bool motif, const dm_config &cfg, uint64_t *his
     p.pre_roots = npre ? d->d_pre_roots : nullptr;
     p.pre_offs = npre ? d->d_pre_offs : nullptr;
     p.pre = npre ? d->d_pre : nullptr;
+    p.pre_cursor = npre ? d->d_pre_cursor : nullptr;
     p.q_group = part->group;
     int queues = 0;
     for (size_t w = 0; w + 1 < sq_off.size(); w++) queues += sq_off[w + 1] > sq_off[w] ? 1 : 0;

@@ -52,6 +52,8 @@ struct DeviceCtx {
     size_t pre_roots_cap = 0;
     int64_t *d_pre_offs = nullptr;
     size_t pre_offs_cap = 0;
+    int32_t *d_pre_cursor = nullptr;
+    size_t pre_cursor_cap = 0;
     char *d_pre = nullptr;
     size_t pre_cap = 0;
 };
@@ -101,7 +103,7 @@ struct dm_graph {
     // static partitions of recent calls (the roots, costs and model they
     // derive from do not change between calls); model_gen counts trainings
     struct Partition {
-        int key[7];
+        int key[8];
         uint64_t model_gen;
         std::vector<int64_t> sq_off;
         std::vector<int32_t> sq_jobs;

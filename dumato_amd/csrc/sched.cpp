@@ -388,14 +388,17 @@ static int predict_all(dm_graph *g, std::vector<float> &pred) {
 // Learned partition.  The model's predicted cost ranks the roots (it is
 // trained on the costs of a smaller k, so its ranking, not its magnitude, is
 // what carries over to the queried k):
-//  * the H = max(64, 5%) highest-ranked roots are cut into pieces of equal
-//    estimated work by first-level child ranges, 64 pieces for the top 32,
-//    32 for the next 96, 16 for the next 384, 8 down to 2% of the roots and
-//    4 for the rest (the ranking of a larger k drifts from the trained
-//    one's, so roots heavy at k can sit lower in it), and each piece
-//    donates from as few as 2 children left; on the clique bitmap path
-//    (k = 4..6, out-degree <= 256) a pre-pass builds their local bitmaps
-//    once, so a piece copies its rows instead of rebuilding them;
+//  * the H = max(64, 5%) highest-ranked roots become P jobs each, 64 for
+//    the top 32, 32 for the next 96, 16 for the next 384, 8 down to 2% of
+//    the roots and 4 for the rest (the ranking of a larger k drifts from the
+//    trained one's, so roots heavy at k can sit lower in it), and each job
+//    donates from as few as 2 children left.  On the clique bitmap path
+//    (k = 4..6, out-degree <= 256) a pre-pass builds the root's local bitmap
+//    once and its P jobs claim children from a shared cursor (guided
+//    self-scheduling: no weight model decides where the root's work lies;
+//    on R-MAT the last children, hubs of the dense core, can hold half of
+//    it).  Elsewhere the P jobs are first-level child ranges of equal
+//    estimated work, child i of n weighing (n - i)^(k-3);
 //  * the lower half of the roots keeps 16 children before it donates (a
 //    light job is not worth rebuilding its prefix elsewhere); the rest keep
 //    the launch's threshold.
@@ -498,6 +501,18 @@ int learned_partition(dm_graph *g, const std::vector<int32_t> &roots, int K, boo
             continue;
         }
         const float lc = kv[idx[rk]] - (float)std::log((double)P);
+        if (bitmap_path && nch <= 256) {
+            // a prebuilt bitmap root: its jobs claim children from one shared
+            // cursor, so no weight model fixes where its work is.  Below k = 6
+            // a root holds about a tenth of the work, and P sharers' row
+            // copies and claims cost more than they balance (R-MAT k = 5:
+            // 1.83 ms with P, 1.46 ms with P / 4)
+            const int PS = K >= 6 ? P : std::max(2, P / 4);
+            part.pre_roots.push_back(v);
+            const float lcs = kv[idx[rk]] - (float)std::log((double)PS);
+            for (int pi = 0; pi < PS; pi++) jobs.push_back({v, -2, PS, q, lcs});
+            continue;
+        }
         const double ws = cw[nch];
         int lo = 0;
         for (int pi = 1; pi <= P && lo < nch; pi++) {
@@ -516,7 +531,6 @@ int learned_partition(dm_graph *g, const std::vector<int32_t> &roots, int K, boo
             jobs.push_back({v, lo, hi >= nch ? INT_MAX : hi, q, lc});
             lo = hi;
         }
-        if (bitmap_path && nch <= 256) part.pre_roots.push_back(v);
     }
     const size_t nheavy = jobs.size();
 

@@ -121,8 +121,10 @@ def test_learned_gain_over_plain_is_measured():
 
 
 def test_learned_pieces_with_prebuilt_bitmaps_rmat():
-    # the learned partition cuts its highest-ranked roots into many pieces
-    # whose local bitmaps a pre-pass builds once; every count stays exact
+    # the learned partition gives its highest-ranked roots many jobs: on the
+    # bitmap path they share the root's prebuilt bitmap and claim children
+    # from a shared cursor, elsewhere they are child ranges.  Every count and
+    # the algorithmic bytes (a root's own bytes counted once) stay exact.
     import json
     import os
     from dumato_amd import workloads as W
@@ -131,8 +133,11 @@ def test_learned_pieces_with_prebuilt_bitmaps_rmat():
     print("case learned rmat train", flush=True)
     g.sched_train(False, 4, epochs=30, seed=1)
     for k in (4, 5, 6):
+        assert g.clique_count(k, dm.config(steal=0)) == gold["clique"][str(k)]
+        want_bytes = g.last_stats()["alg_bytes"]
         for cfg in (dm.config(sched=dm.DM_SCHED_LEARNED), dm.config(sched=dm.DM_SCHED_LEARNED, vwave=32, steal=0),
                     dm.config(sched=dm.DM_SCHED_LEARNED, clique_bitmap=0)):
             print("case learned rmat k=%d vwave %d steal %d bitmap %d" % (k, cfg.vwave, cfg.steal, cfg.clique_bitmap),
                   flush=True)
             assert g.clique_count(k, cfg) == gold["clique"][str(k)]
+            assert g.last_stats()["alg_bytes"] == want_bytes
