@@ -96,6 +96,7 @@ struct dm_graph {
     std::vector<uint64_t> last_costs, last_counts;  // original ids
     dm::CostModel *model = nullptr;
     std::vector<float> features;                    // kMlpIn per degree-rank vertex
+    std::vector<float> root_density;                // per degree-rank vertex: oriented triangles / C(|N+(v)|, 2)
     // autotuned launch parameters per [motif][k]; 0 = not tuned
     int tuned_vwave[2][dm::kMaxK + 1] = {{0}};
     int tuned_lds[2][dm::kMaxK + 1] = {{0}};

@@ -198,9 +198,12 @@ static int compute_features(dm_graph *g) {
     std::vector<int32_t> core;
     core_numbers(c, core);
     g->features.assign((size_t)n * kMlpIn, 0.f);
+    g->root_density.assign(n, 0.f);
     for (int32_t v = 0; v < n; v++) {
         double deg = (double)(c.off[v + 1] - c.off[v]);
         double up = (double)(c.off[v + 1] - g->ostart[v]);
+        g->root_density[v] =
+            (float)std::min(1.0, (double)tri_orig[g->inv[v]] / std::max(1.0, up * (up - 1) / 2));
         double s2 = 0, s2o = 0, mx = 0, s2u = 0;
         for (int64_t e = c.off[v]; e < c.off[v + 1]; e++) {
             int32_t u = c.adj[e];
@@ -441,9 +444,33 @@ int learned_partition(dm_graph *g, const std::vector<int32_t> &roots, int K, boo
     static thread_local std::vector<int32_t> idx, hord;
     static thread_local std::vector<uint16_t> hb;
     kv.resize(nr);
+    // A model trained on size k_train predicts that size's cost.  For a larger
+    // clique size the prediction is carried up by each root's clique growth:
+    // a j-clique of N+(v) extends to about (d - j + 1) / j * rho^(j-1)
+    // (j+1)-cliques, d = |N+(v)| and rho = the density of N+(v) (oriented
+    // triangles / C(d, 2)), both model features.  Without it a k = 4 model
+    // ranks R-MAT's dense-core roots (the heaviest at k = 6) near 2,300th; with
+    // it they are in the top 10 (median of the 20 heaviest).
+    const int kt = g->model->k_train;
+    const bool grow = !motif && K > kt;
+    auto fast_ln = [](double x) {  // ln x to about 0.06, enough to rank
+        int e = 0;
+        const double m = std::frexp(x, &e);  // x = m 2^e, m in [0.5, 1)
+        return ((double)(e - 1) + (2.0 * m - 1.0)) * 0.69314718;
+    };
     float pmin = 1e30f, pmax = -1e30f;
     for (size_t i = 0; i < nr; i++) {
-        kv[i] = pred[roots[i]];
+        const int32_t v = roots[i];
+        double m = pred[v];
+        if (grow) {
+            const double d = (double)(g->rcsr.off[v + 1] - g->ostart[v]);
+            const double rho = (double)g->root_density[v];
+            double gr = 1.0, rp = 1.0;  // rp = rho^(j-1)
+            for (int j = 1; j < kt; j++) rp *= rho;
+            for (int j = kt; j < K; j++, rp *= rho) gr *= std::max(1e-3, (d - j + 1) / j * rp);
+            m += fast_ln(gr);
+        }
+        kv[i] = (float)m;
         pmin = std::min(pmin, kv[i]);
         pmax = std::max(pmax, kv[i]);
     }
