@@ -162,7 +162,12 @@ int dm_pattern_table(int k, uint16_t *quick_to_cls, size_t nquick,
  * the static partition of DuMatoCPU, src/main/DuMatoCPU.cu:121-135) ---- */
 /* Records per-root cycle costs of a plain k_train run on the GPU and trains the
  * 8-64-64-1 MLP cost model on them with the library's HIP kernels.  The model
- * is kept in g and used by later calls with sched = DM_SCHED_LEARNED. */
+ * is kept in g and used by later calls with sched = DM_SCHED_LEARNED: their
+ * partition gives the predicted-heaviest roots many jobs (shared-cursor jobs
+ * over a prebuilt bitmap on the clique bitmap path), orders all jobs heaviest
+ * first and sets per-job donation thresholds.  A clique query larger than
+ * k_train carries the prediction up by each root's clique growth (out-degree
+ * and oriented triangle density). */
 int dm_sched_train(dm_graph *g, int mode_motif, int k_train, int epochs,
                    uint64_t seed, double *final_loss);
 /* Autotuner (no reference counterpart; the reference fixes the virtual-warp
