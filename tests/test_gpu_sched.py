@@ -120,6 +120,34 @@ def test_learned_gain_over_plain_is_measured():
     assert rho_model > 0.3
 
 
+def test_learned_on_small_dense_graphs():
+    # below 64 roots every root is in the heavy head: the top ones get more
+    # jobs than children (capped), bitmap roots get shared-cursor jobs, and
+    # the model (trained at k = 3 or 4) is carried up to k = 6 by the growth
+    # factor; counts and algorithmic bytes equal the oracle's and plain's
+    from tests import graphs as G
+    for seed, (n, p) in enumerate([(40, 0.5), (60, 0.3), (90, 0.15)]):
+        s, d, n = G.gnp(n, p, 900 + seed)
+        g = dm.Graph.from_edges(s, d, n)
+        nv, _, _ = g.info()
+        off, adj = g.csr()
+        ref = O.CSR(nv, off, adj)
+        print("case learned small gnp n=%d p=%.2f" % (n, p), flush=True)
+        g.sched_train(False, 3, epochs=5, seed=seed)
+        for k in (4, 5, 6):
+            want = O.clique_count(ref, k)
+            assert g.clique_count(k, dm.config(steal=0)) == want
+            want_bytes = g.last_stats()["alg_bytes"]
+            for cfg in (dm.config(sched=dm.DM_SCHED_LEARNED), dm.config(sched=dm.DM_SCHED_LEARNED, vwave=16),
+                        dm.config(sched=dm.DM_SCHED_LEARNED, steal=0, vwave=8)):
+                assert g.clique_count(k, cfg) == want
+                assert g.last_stats()["alg_bytes"] == want_bytes
+        g.sched_train(True, 3, epochs=5, seed=seed)
+        wt, wh = O.motif_count(ref, 4)
+        tot, hist = g.motif_count(4, dm.config(sched=dm.DM_SCHED_LEARNED))
+        assert tot == wt and np.array_equal(hist, wh)
+
+
 def test_learned_pieces_with_prebuilt_bitmaps_rmat():
     # the learned partition gives its highest-ranked roots many jobs: on the
     # bitmap path they share the root's prebuilt bitmap and claim children
