@@ -97,6 +97,10 @@ struct dm_graph {
     dm::CostModel *model = nullptr;
     std::vector<float> features;                    // kMlpIn per degree-rank vertex
     std::vector<float> root_density;                // per degree-rank vertex: oriented triangles / C(|N+(v)|, 2)
+    // per degree-rank vertex: log clique growth from size growth_from to
+    // growth_to (the learned partition's extrapolation; cached per sizes)
+    std::vector<float> growth;
+    int growth_from = 0, growth_to = 0;
     // autotuned launch parameters per [motif][k]; 0 = not tuned
     int tuned_vwave[2][dm::kMaxK + 1] = {{0}};
     int tuned_lds[2][dm::kMaxK + 1] = {{0}};

@@ -453,24 +453,24 @@ int learned_partition(dm_graph *g, const std::vector<int32_t> &roots, int K, boo
     // it they are in the top 10 (median of the 20 heaviest).
     const int kt = g->model->k_train;
     const bool grow = !motif && K > kt;
-    auto fast_ln = [](double x) {  // ln x to about 0.06, enough to rank
-        int e = 0;
-        const double m = std::frexp(x, &e);  // x = m 2^e, m in [0.5, 1)
-        return ((double)(e - 1) + (2.0 * m - 1.0)) * 0.69314718;
-    };
-    float pmin = 1e30f, pmax = -1e30f;
-    for (size_t i = 0; i < nr; i++) {
-        const int32_t v = roots[i];
-        double m = pred[v];
-        if (grow) {
+    if (grow && (g->growth_from != kt || g->growth_to != K || g->growth.size() != (size_t)g->rcsr.n)) {
+        // graph-derived like the features: computed once per (k_train, k)
+        g->growth.assign(g->rcsr.n, 0.f);
+        for (int32_t v = 0; v < g->rcsr.n; v++) {
             const double d = (double)(g->rcsr.off[v + 1] - g->ostart[v]);
             const double rho = (double)g->root_density[v];
             double gr = 1.0, rp = 1.0;  // rp = rho^(j-1)
             for (int j = 1; j < kt; j++) rp *= rho;
             for (int j = kt; j < K; j++, rp *= rho) gr *= std::max(1e-3, (d - j + 1) / j * rp);
-            m += fast_ln(gr);
+            g->growth[v] = (float)std::log(gr);
         }
-        kv[i] = (float)m;
+        g->growth_from = kt;
+        g->growth_to = K;
+    }
+    float pmin = 1e30f, pmax = -1e30f;
+    for (size_t i = 0; i < nr; i++) {
+        const int32_t v = roots[i];
+        kv[i] = grow ? pred[v] + g->growth[v] : pred[v];
         pmin = std::min(pmin, kv[i]);
         pmax = std::max(pmax, kv[i]);
     }
@@ -595,20 +595,27 @@ int learned_partition(dm_graph *g, const std::vector<int32_t> &roots, int K, boo
     part.sq_off.assign(nq + 1, 0);
     for (size_t qi = 0; qi < nq; qi++) part.sq_off[qi + 1] = part.sq_off[qi] + (int64_t)((J + nq - 1 - qi) / nq);
     part.sq_jobs.resize(J * 4);
-    auto put = [&](size_t t, int32_t root, int32_t lo, int32_t hi, int32_t q) {
-        int32_t *jp = &part.sq_jobs[4 * (size_t)(part.sq_off[t % nq] + (int64_t)(t / nq))];
+    // list positions in order: queue qi = t % nq, slot = t / nq, kept as
+    // counters (no division per job)
+    size_t qi = 0, slot = 0;
+    auto put = [&](int32_t root, int32_t lo, int32_t hi, int32_t q) {
+        int32_t *jp = &part.sq_jobs[4 * (size_t)(part.sq_off[qi] + (int64_t)slot)];
         jp[0] = root;
         jp[1] = lo;
         jp[2] = hi;
         jp[3] = q;
+        if (++qi == nq) {
+            qi = 0;
+            slot++;
+        }
     };
     for (size_t t = 0; t < nheavy; t++) {
         const Job &jb = jobs[(size_t)hord[t]];
-        put(t, jb.root, jb.lo, jb.hi, jb.q);
+        put(jb.root, jb.lo, jb.hi, jb.q);
     }
     const bool halves = nr > 2 * H;
     for (size_t t = H; t < nr; t++)
-        put(nheavy + (t - H), roots[idx[t]], 0, INT_MAX, halves && t >= nr / 2 ? make_job_query(0, 16) : 0);
+        put(roots[idx[t]], 0, INT_MAX, halves && t >= nr / 2 ? make_job_query(0, 16) : 0);
 
     // prebuilt bitmaps, ascending by root
     std::sort(part.pre_roots.begin(), part.pre_roots.end());
