@@ -31,6 +31,7 @@ struct CostModel {
     int mode_motif = 0, k_train = 0;
     double final_loss = 0;
     float *d_params = nullptr, *d_x = nullptr, *d_out = nullptr;  // inference buffers
+    float *h_out = nullptr;  // pinned host copy of the standardised outputs (n floats)
 };
 
 }  // namespace dm

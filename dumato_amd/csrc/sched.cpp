@@ -130,6 +130,7 @@ void free_model(dm_graph *g) {
         if (g->model->d_params) (void)hipFree(g->model->d_params);
         if (g->model->d_x) (void)hipFree(g->model->d_x);
         if (g->model->d_out) (void)hipFree(g->model->d_out);
+        if (g->model->h_out) (void)hipHostFree(g->model->h_out);
     }
     delete g->model;
     g->model = nullptr;
@@ -362,7 +363,10 @@ static int train_model(const std::vector<float> &X, const std::vector<float> &Y,
 // Predicted log-cycles per degree-rank vertex.  The standardised features
 // and the parameters stay on the device after the first call; each call is
 // one forward launch and one copy back.
-static int predict_all(dm_graph *g, std::vector<float> &pred) {
+// One forward launch over every vertex; *out (n floats, degree-rank ids) holds
+// the standardised outputs until the next call: predicted log cycles are
+// out[v] * y_std + y_mean.
+static int predict_all(dm_graph *g, const float **out) {
     CostModel &M = *g->model;
     const int32_t n = g->rcsr.n;
     DeviceCtx *ctx = g->dev->ctx;
@@ -373,17 +377,17 @@ static int predict_all(dm_graph *g, std::vector<float> &pred) {
             for (int i = 0; i < kMlpIn; i++)
                 X[(size_t)v * kMlpIn + i] = (g->features[(size_t)v * kMlpIn + i] - M.feat_mean[i]) / M.feat_std[i];
         if (hipMalloc(&M.d_params, kMlpParams * 4) != hipSuccess ||
-            hipMalloc(&M.d_x, X.size() * 4 + 4) != hipSuccess || hipMalloc(&M.d_out, (size_t)n * 4 + 4) != hipSuccess)
+            hipMalloc(&M.d_x, X.size() * 4 + 4) != hipSuccess || hipMalloc(&M.d_out, (size_t)n * 4 + 4) != hipSuccess ||
+            hipHostMalloc((void **)&M.h_out, (size_t)n * 4 + 4, hipHostMallocDefault) != hipSuccess)
             return set_error(DM_ERR_NOMEM, "MLP inference buffers");
         HIPCHK(hipMemcpy(M.d_params, M.params.data(), kMlpParams * 4, hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(M.d_x, X.data(), X.size() * 4, hipMemcpyHostToDevice));
     }
     int r = mlp_forward_launch(M.d_params, M.d_x, n, M.d_out, ctx->stream);
     if (r) return r;
-    pred.resize(n);
-    HIPCHK(hipMemcpyAsync(pred.data(), M.d_out, (size_t)n * 4, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(M.h_out, M.d_out, (size_t)n * 4, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
-    for (auto &p : pred) p = p * M.y_std + M.y_mean;
+    *out = M.h_out;
     return DM_OK;
 }
 
@@ -416,8 +420,9 @@ int learned_partition(dm_graph *g, const std::vector<int32_t> &roots, int K, boo
     const auto t0 = clk::now();
     int r = compute_features(g);
     if (r) return r;
-    std::vector<float> pred;
-    if ((r = predict_all(g, pred))) return r;
+    const float *z = nullptr;  // standardised predictions
+    if ((r = predict_all(g, &z))) return r;
+    const float ys = g->model->y_std, ym = g->model->y_mean;
     const auto t1 = clk::now();
     g->last_infer_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
     struct BinTimer {
@@ -470,7 +475,8 @@ int learned_partition(dm_graph *g, const std::vector<int32_t> &roots, int K, boo
     float pmin = 1e30f, pmax = -1e30f;
     for (size_t i = 0; i < nr; i++) {
         const int32_t v = roots[i];
-        kv[i] = grow ? pred[v] + g->growth[v] : pred[v];
+        const float lc = z[v] * ys + ym;  // predicted log cycles at the training size
+        kv[i] = grow ? lc + g->growth[v] : lc;
         pmin = std::min(pmin, kv[i]);
         pmax = std::max(pmax, kv[i]);
     }
@@ -727,9 +733,10 @@ extern "C" int dm_sched_predict(dm_graph *g, float *pred, size_t n) {
     int r = ensure_device(g);
     if (r) return r;
     if ((r = compute_features(g))) return r;
-    std::vector<float> p;
-    if ((r = predict_all(g, p))) return r;
-    for (int32_t v = 0; v < g->rcsr.n; v++) pred[g->inv[v]] = p[v];
+    const float *z = nullptr;
+    if ((r = predict_all(g, &z))) return r;
+    const CostModel &M = *g->model;
+    for (int32_t v = 0; v < g->rcsr.n; v++) pred[g->inv[v]] = z[v] * M.y_std + M.y_mean;
     return DM_OK;
 }
 
