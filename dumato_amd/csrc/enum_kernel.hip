@@ -933,6 +933,12 @@ struct Worker {
         uint64_t *rows = (uint64_t *)alloc((int64_t)d * NW * 8);
         int32_t *dp = (int32_t *)alloc((int64_t)d * 4);
         int32_t *pre = (int32_t *)alloc((int64_t)(d + 1) * 4);
+        uint64_t *s2 = nullptr;  // k = 6: a round's pair staging
+        int32_t *sp = nullptr;
+        if (K == 6) {
+            s2 = (uint64_t *)alloc((int64_t)W * NW * 8);
+            sp = (int32_t *)alloc((int64_t)(W + 1) * 4);
+        }
         if (abort) return;
         // a root whose bitmap the launch prebuilt (a heavy root cut into
         // many pieces): copy its rows instead of building them
@@ -980,11 +986,11 @@ struct Worker {
                 n = g.bcast(n, 0);
                 if (a >= d) break;
                 if (a == 0 && g.gl == 0) bytes += 4ull * (unsigned long long)d;
-                pair_phase<NW>(R, rows, dp, pre, a, a + n < d ? a + n : d);
+                pair_phase<NW>(R, rows, dp, pre, s2, sp, a, a + n < d ? a + n : d);
             }
         } else {
             if (first_piece && g.gl == 0) bytes += 4ull * (unsigned long long)d;
-            pair_phase<NW>(R, rows, dp, pre, lo, hi);
+            pair_phase<NW>(R, rows, dp, pre, s2, sp, lo, hi);
         }
         lds_top = R.mark_lds;
         g_top = R.mark_glb;
@@ -1075,7 +1081,10 @@ struct Worker {
     // first-level children [lo, hi) in batches of W; each batch is
     // flattened into its (i, j) pairs, which the lanes stride over
     template <int NW>
-    DEV void pair_phase(Lvl &R, const uint64_t *rows, const int32_t *dp, int32_t *pre, int lo, int hi) {
+    // s2 / sp (k = 6 only): LDS staging of a round's W pairs, W x NW words
+    // and W + 1 prefix entries
+    DEV void pair_phase(Lvl &R, const uint64_t *rows, const int32_t *dp, int32_t *pre, uint64_t *s2, int32_t *sp,
+                        int lo, int hi) {
         R.cur = lo;
         R.end = hi;
         unsigned long long by = 0;
@@ -1113,27 +1122,94 @@ struct Worker {
             if (g.gl == 0) pre[0] = 0;
             const int P = g.bcast(sc, take - 1);
             wsync();
+            if (K == 6) {
+                pair_round_k6<NW>(rows, dp, pre, s2, sp, cur, P, by);
+                continue;
+            }
             int a = 0;
             for (int t = g.gl; t < P; t += W) {
                 while (pre[a + 1] <= t) a++;
                 const int i = cur + a;
-                int r = t - pre[a];
-                int w = 0;
-                uint64_t m = rows[i * NW];
-                while (true) {
-                    const int c = __popcll(m);
-                    if (r < c || w == NW - 1) break;
-                    r -= c;
-                    w++;
-                    m = rows[i * NW + w];
-                }
-                const int j = w * 64 + select64(m, r);
+                const int j = pair_second<NW>(rows, i, t - pre[a]);
                 by += (unsigned long long)dp[j];
                 bitmap_pair<NW>(rows, dp, i, j, cnt, by);
             }
             wsync();
         }
         bytes += 4ull * by;
+    }
+
+    // the second child of pair r of child i: the r-th set bit of row i
+    template <int NW>
+    DEV static int pair_second(const uint64_t *rows, int i, int r) {
+        int w = 0;
+        uint64_t m = rows[i * NW];
+        while (true) {
+            const int c = __popcll(m);
+            if (r < c || w == NW - 1) break;
+            r -= c;
+            w++;
+            m = rows[i * NW + w];
+        }
+        return w * 64 + select64(m, r);
+    }
+
+    // k = 6: the pairs of a batch, W at a time.  Each lane stages its pair's
+    // common candidates S2 = row i & row j in LDS, then the lanes stride over
+    // the round's (pair, k in S2) triples, each counting the 2-cliques of
+    // S2 & row k above k: one level finer than a lane per pair, whose cost
+    // (|S2| x |S3|) varies too much across the lanes of a wave.
+    template <int NW>
+    DEV void pair_round_k6(const uint64_t *rows, const int32_t *dp, const int32_t *pre, uint64_t *s2, int32_t *sp,
+                           int cur, int P, unsigned long long &by) {
+        int a = 0;
+        for (int t0 = 0; t0 < P; t0 += W) {
+            const int t = t0 + g.gl;
+            int s = 0;
+            if (t < P) {
+                while (pre[a + 1] <= t) a++;
+                const int i = cur + a;
+                const int j = pair_second<NW>(rows, i, t - pre[a]);
+                by += (unsigned long long)dp[j];
+#pragma unroll
+                for (int w = 0; w < NW; w++) {
+                    const uint64_t x = rows[i * NW + w] & rows[j * NW + w];
+                    s2[g.gl * NW + w] = x;
+                    s += __popcll(x);
+                }
+            }
+            const int ps = g.iscan(s);
+            sp[g.gl + 1] = ps;
+            if (g.gl == 0) sp[0] = 0;
+            const int Ts = g.bcast(ps, W - 1);
+            wsync();
+            int pa = 0;
+            unsigned long long c = 0;
+            for (int u = g.gl; u < Ts; u += W) {
+                while (sp[pa + 1] <= u) pa++;
+                const uint64_t *S2 = s2 + pa * NW;
+                const int k = pair_second<NW>(S2, 0, u - sp[pa]);
+                by += (unsigned long long)dp[k];
+                uint64_t S3[NW];
+#pragma unroll
+                for (int w = 0; w < NW; w++) S3[w] = S2[w] & rows[k * NW + w];
+                int pc = 0;
+#pragma unroll
+                for (int w3 = 0; w3 < NW; w3++) {
+                    uint64_t m3 = S3[w3];
+                    while (m3) {
+                        const int q = w3 * 64 + __builtin_ctzll(m3);
+                        m3 &= m3 - 1;
+                        by += (unsigned long long)dp[q];
+#pragma unroll
+                        for (int w = 0; w < NW; w++) pc += __popcll(S3[w] & rows[q * NW + w]);
+                    }
+                }
+                c += (unsigned long long)pc;
+            }
+            cnt += c;
+            wsync();
+        }
     }
 
     // ---- one job ------------------------------------------------------------

@@ -284,6 +284,8 @@ int64_t arena_bound_for(int32_t maxdeg, int32_t max_out_, int K, bool motif) {
         // local bitmap of a root (out-degree <= 256): rows, out-degrees,
         // prefix, then the build's hash (512 keys + positions) and list starts
         b += 256 * 4 * 8 + 2 * 257 * 4 + 512 * 5 + 256 * 8 + 6 * 16;
+        // k = 6: a round's pair staging (64 lanes x 4 words, 65 prefix entries)
+        if (K == 6) b += 64 * 4 * 8 + 65 * 4 + 2 * 16;
     }
     return (b + 255) & ~(int64_t)255;
 }
