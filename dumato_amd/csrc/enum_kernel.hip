@@ -36,6 +36,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 
 #include "internal.h"
 
@@ -933,12 +934,6 @@ struct Worker {
         uint64_t *rows = (uint64_t *)alloc((int64_t)d * NW * 8);
         int32_t *dp = (int32_t *)alloc((int64_t)d * 4);
         int32_t *pre = (int32_t *)alloc((int64_t)(d + 1) * 4);
-        uint64_t *s2 = nullptr;  // k = 6: a round's pair staging
-        int32_t *sp = nullptr;
-        if (K == 6) {
-            s2 = (uint64_t *)alloc((int64_t)W * NW * 8);
-            sp = (int32_t *)alloc((int64_t)(W + 1) * 4);
-        }
         if (abort) return;
         // a root whose bitmap the launch prebuilt (a heavy root cut into
         // many pieces): copy its rows instead of building them
@@ -963,6 +958,16 @@ struct Worker {
             return;
         } else {
             build_rows<NW>(v, lo, d, L, rows, dp, pre);
+            if (abort) return;
+        }
+        // k = 6: a round's pair staging, allocated after the build so that
+        // the build's hash gets the LDS first (both fit there together only
+        // for smaller roots)
+        uint64_t *s2 = nullptr;
+        int32_t *sp = nullptr;
+        if (K == 6) {
+            s2 = (uint64_t *)alloc((int64_t)W * NW * 8);
+            sp = (int32_t *)alloc((int64_t)(W + 1) * 4);
             if (abort) return;
         }
         if (shared) {
@@ -1655,8 +1660,11 @@ __global__ __launch_bounds__(kThreads, (min_waves<W, MODE>())) void enum_kernel(
     }
     // Per-worker statistics (held by lane 0 of each worker) reduced over the
     // block, so that each block, not each worker, adds to the shared words.
+    // The reduction reuses the start of the block's dynamic LDS (at least the
+    // worker records, > 80 B per wave): a static array would count towards
+    // the LDS that decides how many blocks share a CU.
     constexpr int NF = 10;
-    __shared__ unsigned long long red[kThreads / 64][NF];
+    unsigned long long (*red)[NF] = (unsigned long long (*)[NF])smem;
     const bool holder = wid < p.nworkers && g.gl == 0;
     const unsigned long long *st = ws.st();
     unsigned long long f[NF] = {holder ? bytes : 0ull,          holder ? st[kStJobs] : 0ull,
@@ -1673,6 +1681,9 @@ __global__ __launch_bounds__(kThreads, (min_waves<W, MODE>())) void enum_kernel(
             f[i] = i < 5 ? f[i] + y : (y > f[i] ? y : f[i]);
         }
     }
+    // every wave has read its records (and the class counters were flushed)
+    // before the reduction overwrites them
+    __syncthreads();
     if (__lane_id() == 0) {
 #pragma unroll
         for (int i = 0; i < NF; i++) red[threadIdx.x / 64][i] = f[i];
@@ -1879,7 +1890,25 @@ int enum_occupancy(int vwave, int mode, int threads, size_t lds, int *blocks_per
     int nb = 0;
     hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k, threads, lds);
     if (e != hipSuccess) return set_error(DM_ERR_HIP, "occupancy query: %s", hipGetErrorString(e));
-    *blocks_per_cu = nb;
+    // The query divides the CU's 160 KiB by the dynamic size alone.  gfx950
+    // allocates a block's LDS, static and dynamic together, in 1280-byte
+    // units (measured: 53,760 B per block keeps 3 blocks on a CU, 53,761 B
+    // leaves 2), so a persistent launch sized by the query alone would hold
+    // blocks that only become resident once others have exited.
+    constexpr size_t kLdsGranule = 1280;
+    static std::atomic<int> static_lds[4][3] = {};  // + 1: 0 = not read yet
+    const int vi = vwave == 8 ? 0 : vwave == 16 ? 1 : vwave == 32 ? 2 : 3;
+    int sl = static_lds[vi][mode].load(std::memory_order_relaxed) - 1;
+    if (sl < 0) {
+        hipFuncAttributes fa{};
+        e = hipFuncGetAttributes(&fa, k);
+        if (e != hipSuccess) return set_error(DM_ERR_HIP, "hipFuncGetAttributes: %s", hipGetErrorString(e));
+        sl = (int)fa.sharedSizeBytes;
+        static_lds[vi][mode].store(sl + 1, std::memory_order_relaxed);
+    }
+    const size_t per_block = (lds + (size_t)sl + kLdsGranule - 1) / kLdsGranule * kLdsGranule;
+    const int fit = (int)((size_t)160 * 1024 / per_block);
+    *blocks_per_cu = nb < fit ? nb : fit;
     return DM_OK;
 }
 

@@ -229,10 +229,12 @@ static int max_lds_uncached(int vwave, int threads, int mode, int K, int ncls_si
     if (enum_occupancy(vwave, mode, threads, fixed, &bpc) != DM_OK || bpc < 1) return 32768 / wpb;
     const size_t per_block = (size_t)160 * 1024 / (size_t)bpc;
     if (per_block <= fixed + 256) return std::min(32768 / wpb, 16384);
-    int lds = std::min((int)(((per_block - fixed) / (size_t)wpb) & ~(size_t)255), 16384);
+    // 16-byte steps: the bitmap path's rows and build hash at out-degree 256
+    // need all but a few bytes of a worker's share at 3 blocks per CU
+    int lds = std::min((int)(((per_block - fixed) / (size_t)wpb) & ~(size_t)15), 16384);
     // the LDS allocation granularity may cost a block: step down until the
     // occupancy holds
-    for (; lds > 256; lds -= 256) {
+    for (; lds > 256; lds -= 16) {
         int b2 = 0;
         const size_t total = enum_lds_bytes(vwave, threads, K, ncls_single, hist_slots, lds);
         if (enum_occupancy(vwave, mode, threads, total, &b2) == DM_OK && b2 >= bpc) break;
