@@ -13,8 +13,9 @@ Same JSON line, extra blocks (--no-extra skips the first three):
   motif5_ba       k=5 motifs of the Barabasi-Albert n=100000 graph with 200
                   planted cliques.
 The last two compare the plain runtime at its default launch shape (full
-64-lane waves, default LDS stack, steal granularity 2), the plain runtime
-autotuned, and the learned scheduler autotuned and at the default shape.  The
+64-lane waves, default LDS stack, default steal granularity: 2 for motifs,
+32 for cliques), the plain runtime autotuned, and the learned scheduler
+autotuned and at the default shape.  The
 cost model is trained on per-root cycle counts of k=4 runs of the same graph;
 for k=6 the learned query is also run with the model retrained on k=5 costs
 (reported beside it, *_train_k_minus_1).  Query time = kernel + host scheduling, and every
@@ -219,7 +220,7 @@ def plain_vs_learned(graph, mode, k_list, warmup, steps, k_train=4, budget_ms=30
     motif = mode == "motif"
     for k in k_list:
         # the default shape, fixed before any tuning is cached in the graph
-        out["k%d" % k] = {"plain_default": query_block(graph, mode, k, dm.config(vwave=64, steal_min=2, repartition=1),
+        out["k%d" % k] = {"plain_default": query_block(graph, mode, k, dm.config(vwave=64, repartition=1),
                                                        warmup, steps)}
         progress("%s k=%d plain default shape: kernel %.2f ms" % (mode, k, out["k%d" % k]["plain_default"]["kernel_ms"]))
 
@@ -244,7 +245,7 @@ def plain_vs_learned(graph, mode, k_list, warmup, steps, k_train=4, budget_ms=30
         progress("%s k=%d learned tuned: kernel %.2f ms, sched %.2f ms" %
                  (mode, k, rec["learned_tuned"]["kernel_ms"], rec["learned_tuned"]["sched_ms"]))
         # the learned partition at the plain default's own shape
-        rec["learned_default"] = query_block(graph, mode, k, dm.config(vwave=64, steal_min=2, repartition=1,
+        rec["learned_default"] = query_block(graph, mode, k, dm.config(vwave=64, repartition=1,
                                                                        sched=dm.DM_SCHED_LEARNED), warmup, steps)
         progress("%s k=%d learned default shape: kernel %.2f ms" % (mode, k, rec["learned_default"]["kernel_ms"]))
         pd, pt, lt, ld = rec["plain_default"], rec["plain_tuned"], rec["learned_tuned"], rec["learned_default"]

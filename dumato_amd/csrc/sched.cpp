@@ -32,9 +32,12 @@ int autotuned_vwave(dm_graph *g, int K, bool motif) {
     return t ? t : 64;
 }
 
+// Untuned donation threshold: 2 for motifs; 32 for cliques, whose bitmap
+// roots rebuild part of their bitmap for every donated piece (R-MAT s17
+// k = 5: 2.23 ms at 2, 1.43 ms at 32, 1.82 ms at 256).
 int tuned_steal_min(const dm_graph *g, int K, bool motif) {
     int t = g->tuned_steal_min[motif ? 1 : 0][K];
-    return t ? t : 2;
+    return t ? t : motif ? 2 : 32;
 }
 
 int tuned_lds(const dm_graph *g, int K, bool motif, int vwave, int fallback) {
@@ -80,7 +83,7 @@ int autotune(dm_graph *g, int K, bool motif, double budget_ms, dm_config *best_o
     double ms = 0;
     if ((r = trial(base, S, ms))) return r;
     // too slow for the budget: fewer candidates rather than a thinner sample
-    const bool full_search = ms * 14 <= budget_ms;
+    const bool full_search = ms * 18 <= budget_ms;
     const bool width_search = ms * 6 <= budget_ms;
     double best = ms;
     dm_config bc = base;
@@ -101,7 +104,7 @@ int autotune(dm_graph *g, int K, bool motif, double budget_ms, dm_config *best_o
     if (full_search) {
         const int d8 = 32768 / (256 / bc.vwave);  // 8 KiB for a full wave
         for (int lds : {(dl / 2) & ~255, std::min(d8, dl), dl}) {
-            for (int sm : {2, 8}) {
+            for (int sm : {2, 8, 32, 256}) {
                 dm_config c = bc;
                 c.lds_bytes = lds;
                 c.steal_min = sm;
