@@ -65,6 +65,8 @@ struct dm_batch {
     size_t sq_jobs_cap = 0;
     int32_t *d_sq_cur = nullptr;
     size_t sq_cur_cap = 0;
+    unsigned long long *d_qmask = nullptr;  // non-empty static queues (shared by the phases)
+    size_t qmask_cap = 0;
     unsigned long long *d_results = nullptr;
     size_t results_cap = 0;
     dm::QueryDesc *d_qd = nullptr;
@@ -282,6 +284,7 @@ static int batch_build(dm_batch *b, const std::vector<dm_query> &qs, const dm_co
     if ((r = dev_grow(b->d_sq_off, b->sq_off_cap, sq_total))) return r;
     if ((r = dev_grow(b->d_sq_jobs, b->sq_jobs_cap, std::max<size_t>(jobs_total, 4)))) return r;
     if ((r = dev_grow(b->d_sq_cur, b->sq_cur_cap, (size_t)max_workers))) return r;
+    if ((r = dev_grow(b->d_qmask, b->qmask_cap, (size_t)qmask_words(max_workers, 1)))) return r;
     if ((r = dev_grow(b->d_results, b->results_cap, b->nres))) return r;
     if ((r = dev_grow(b->d_qd, b->qd_cap, qs.size()))) return r;
     if ((r = dev_grow(b->d_done, b->done_cap, qs.size()))) return r;
@@ -326,6 +329,7 @@ static int batch_build(dm_batch *b, const std::vector<dm_query> &qs, const dm_co
         p.sq_off = b->d_sq_off + o_off;
         p.sq_jobs = b->d_sq_jobs + o_jobs;
         p.sq_cur = b->d_sq_cur;
+        p.qmask = P.mode == 1 ? b->d_qmask : nullptr;  // motif phases only, as in single-graph calls
         p.q = d->d_q;
         p.q_cap = (uint32_t)d->q_cap_alloc;
         p.epoch = d->d_epoch;
@@ -614,7 +618,7 @@ int dm_batch_num_results(const dm_batch *b, const dm_query *queries, size_t nq, 
 void dm_batch_free(dm_batch *b) {
     if (!b) return;
     drop_graph(b);
-    void *bufs[] = {b->d_arena, b->d_sq_off, b->d_sq_jobs, b->d_sq_cur, b->d_results,
+    void *bufs[] = {b->d_arena, b->d_sq_off, b->d_sq_jobs, b->d_sq_cur, b->d_qmask, b->d_results,
                     b->d_qd, b->d_done, b->d_start, b->d_sc};
     for (void *p : bufs)
         if (p) (void)hipFree(p);

@@ -1379,7 +1379,10 @@ DEV bool take_static(const EnumParams &p, const Grp<W> &g, int w, const WorkerLD
                     q = t[3];
                 }
                 // whoever claims a queue's last job retires the queue
-                if (b + j == e - 1) atomicSub(&p.sc->static_left, 1);
+                if (b + j == e - 1) {
+                    if (p.qmask) atomicAnd(&p.qmask[w >> 6], ~(1ull << (w & 63)));
+                    atomicSub(&p.sc->static_left, 1);
+                }
             }
         }
     }
@@ -1474,6 +1477,39 @@ DEV bool steal_static(const EnumParams &p, const Grp<W> &g, int w, const WorkerL
     }
     const int nq = num_queues(p);
     if (nq <= 1) return false;  // the one queue was tried by take_static
+    if (p.qmask) {
+        // a random queue among those still holding jobs: the lanes read the
+        // mask words from a random word on, the first non-zero word (in that
+        // order) gives a random set bit.  Plain reads of rarely written
+        // words; thieves spread over the non-empty queues.
+        const int nwords = (nq + 63) >> 6;
+        for (int attempt = 0; attempt < 4; attempt++) {
+            seed = seed * 1664525u + 1013904223u;
+            const int start = (int)((seed >> 8) % (uint32_t)nwords);
+            const int rot = (int)((seed >> 20) & 63);
+            int v = -1;
+            for (int base = 0; base < nwords && v < 0; base += W) {
+                const int i = base + g.gl;
+                int wi = start + i;
+                if (wi >= nwords) wi -= nwords;
+                unsigned long long m = 0;
+                if (i < nwords) m = __hip_atomic_load(&p.qmask[wi], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                const uint64_t bal = g.ballot(m != 0);
+                if (bal) {
+                    const int l = __builtin_ctzll(bal);
+                    int q = 0;
+                    if (g.gl == l) {
+                        const unsigned long long r = rot ? (m >> rot) | (m << (64 - rot)) : m;
+                        q = wi * 64 + ((__builtin_ctzll(r) + rot) & 63);
+                    }
+                    v = g.bcast(q, l);
+                }
+            }
+            if (v < 0) return false;  // every queue retired or being retired
+            if (v != w && v < nq && take_static<W>(p, g, v, ws, jh)) return true;
+        }
+        return false;
+    }
     // random probes: idle workers spread over the queues (a scan for the
     // fullest queue sends them all to the same cursor, and device-scope
     // atomics on one address slow every worker's memory traffic)
@@ -1818,7 +1854,8 @@ __global__ void fill_u32_kernel(uint32_t *ptr, uint32_t value, size_t count) {
 // match.
 __global__ void enum_init_kernel(SchedCounters *sc, unsigned long long njobs, int32_t queues, int32_t *epoch, int32_t *sq_cur,
                                  int nworkers, unsigned long long *results, size_t nresults,
-                                 unsigned long long *done_ticks, int nq, unsigned long long *start_ticks) {
+                                 unsigned long long *done_ticks, int nq, unsigned long long *start_ticks,
+                                 unsigned long long *qmask, int nqueues, const int64_t *sq_off, int strided_n) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     size_t stride = (size_t)gridDim.x * blockDim.x;
     if (i == 0) {
@@ -1845,6 +1882,20 @@ __global__ void enum_init_kernel(SchedCounters *sc, unsigned long long njobs, in
     for (size_t t = i; t < nresults; t += stride) results[t] = 0;
     if (done_ticks)
         for (size_t t = i; t < (size_t)nq; t += stride) done_ticks[t] = 0;
+    // non-empty static queues (strided partition: queue w holds roots iff w < n)
+    if (qmask) {
+        const size_t nwords = ((size_t)nqueues + 63) / 64;
+        for (size_t t = i; t < nwords; t += stride) {
+            unsigned long long m = 0;
+            for (int b = 0; b < 64; b++) {
+                const int64_t q = (int64_t)t * 64 + b;
+                if (q >= nqueues) break;
+                const bool ne = sq_off ? sq_off[q + 1] > sq_off[q] : q < strided_n;
+                m |= (unsigned long long)ne << b;
+            }
+            qmask[t] = m;
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1927,8 +1978,11 @@ int enum_init_launch(const EnumParams &p, unsigned long long njobs, int queues, 
                      void *stream) {
     size_t work = std::max<size_t>({(size_t)p.nworkers, nresults, (size_t)nq, (size_t)1});
     int blocks = (int)std::min<size_t>((work + 255) / 256, 1024);
+    const int g = p.q_group > 1 ? p.q_group : 1;
+    const int nqueues = p.sq_jobs ? (p.nworkers + g - 1) / g : p.nworkers;
     hipLaunchKernelGGL(enum_init_kernel, dim3(blocks), dim3(256), 0, (hipStream_t)stream, p.sc, njobs, queues, epoch,
-                       p.sq_cur, p.nworkers, results, nresults, done_ticks, nq, p.start_ticks);
+                       p.sq_cur, p.nworkers, results, nresults, done_ticks, nq, p.start_ticks, p.qmask, nqueues,
+                       p.sq_jobs ? p.sq_off : (const int64_t *)nullptr, p.n);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return set_error(DM_ERR_HIP, "init launch: %s", hipGetErrorString(e));
     return DM_OK;

@@ -173,7 +173,18 @@ struct EnumParams {
     // claims from queue w / q_group of the (nworkers + q_group - 1) / q_group
     // queues, and an idle worker probes the other queues
     int32_t q_group;
+    int32_t pad6;
+    // nullable: one bit per static queue, set while the queue holds an
+    // unclaimed job (set by the init kernel, cleared by the claimer of the
+    // queue's last job); idle workers steal from a set bit's queue
+    unsigned long long *qmask;
 };
+
+// 64-bit words of the non-empty-queue mask of a launch with these workers
+inline int qmask_words(int nworkers, int q_group) {
+    const int g = q_group > 1 ? q_group : 1;
+    return ((nworkers + g - 1) / g + 63) / 64;
+}
 
 // A static or donated job's query word: bits 0..15 the query index, bits
 // 16..23 the job's own donation threshold (0: the launch's steal_min).

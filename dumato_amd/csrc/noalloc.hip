@@ -44,13 +44,14 @@ constexpr size_t kSoLut = 32 << 10;                   // class table (uint16)
 constexpr size_t kLutBytes = (size_t)2 << 20;         // 2^20 entries: k <= 7
 constexpr size_t kSoCur = kSoLut + kLutBytes;         // static-queue cursors
 constexpr size_t kMaxWorkers = 65536;
+constexpr size_t kSoQmask = kSoRes + kResSlots * 8;    // non-empty static queues (1 bit per worker)
 constexpr size_t kSoQueue = kSoCur + kMaxWorkers * 4;  // donated-job queue
 constexpr uint32_t kQueueSlots = 16384;
 constexpr size_t kSoArena = kSoQueue + (size_t)kQueueSlots * sizeof(Job);
 constexpr size_t kArenaBytes = kScratchBytes - kSoArena;
 static_assert(sizeof(SchedCounters) <= kSoEpoch - kSoSc, "counters overlap the epoch");
 static_assert(sizeof(QueryDesc) <= kSoRes - kSoQd, "query descriptor overlaps the results");
-static_assert(kSoRes + kResSlots * 8 <= kSoLut, "results overlap the class table");
+static_assert(kSoQmask + kMaxWorkers / 8 <= kSoLut, "queue mask overlaps the class table");
 static_assert(kSoQueue % 256 == 0 && kSoArena % 256 == 0, "misaligned scratch regions");
 
 // Device address of `scratch` per device, taken once per device with the
@@ -170,6 +171,7 @@ static int noalloc_count(dm_graph *g, bool motif, int K, const dm_config &cfg, u
     p.sq_off = nullptr;
     p.sq_jobs = nullptr;  // strided static partition
     p.sq_cur = (int32_t *)(sp + kSoCur);
+    p.qmask = motif ? (unsigned long long *)(sp + kSoQmask) : nullptr;  // motifs only, as in dm_count
     p.q = (Job *)(sp + kSoQueue);
     p.q_cap = kQueueSlots;
     p.epoch = (const int32_t *)(sp + kSoEpoch);

@@ -42,6 +42,8 @@ struct DeviceCtx {
     size_t sq_jobs_cap = 0;
     int32_t *d_sq_cur = nullptr;
     size_t sq_cur_cap = 0;
+    unsigned long long *d_qmask = nullptr;  // non-empty static queues
+    size_t qmask_cap = 0;
     SchedCounters *d_sc = nullptr;
     unsigned long long *d_results = nullptr;  // clique count or motif histogram
     size_t results_cap = 0;
