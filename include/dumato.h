@@ -43,7 +43,8 @@ typedef struct dm_config {
     int32_t vwave;           /* lanes per DFS worker: 8, 16, 32, 64; 0 = autotuned */
     int32_t steal;           /* 1: idle workers receive donated subtrees */
     int32_t steal_min;       /* donate only levels with >= steal_min children left
-                                (the last level: >= 4 * steal_min); 0 = autotuned */
+                                (the last level: >= 4 * steal_min); 0 = autotuned
+                                (untuned: 2 for motifs, 32 for cliques) */
     int32_t steal_period;    /* microseconds between a worker's checks for idle
                                 workers (0: every DFS step) */
     int32_t lds_bytes;       /* LDS stack bytes per worker; 0 = autotuned */
