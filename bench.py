@@ -32,7 +32,8 @@ roofline: the resource that bounds the headline kernel.  Its shader counters
 profiles/enum_kernel_motif4_chunglu18k.md) show VALU issue as the most used
 unit (latency-bound otherwise: waves wait about two thirds of their cycles),
 so achieved = VALU wave-instructions per launch (profile) / kernel time (HIP
-events on the library's stream, this run), against the MI355X issue peak of
+events on the library's stream, this run; frac_enumeration leaves out the idle
+workers' steal loop, 9% of the count), against the MI355X issue peak of
 256 CUs x 4 SIMDs x 2.4 GHz / 4 cycles per wave64 instruction.  Beside it:
 traffic = HBM bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, calibrated
 on a copy of known size) and the algorithmic bytes of the enumeration
@@ -319,13 +320,18 @@ def roofline(st, kern_ms, total):
     if os.path.exists(PROFILE_COUNTERS):
         rec = json.load(open(PROFILE_COUNTERS)).get("motif4_chunglu18k", {})
     valu = rec.get("valu_insts")
+    steal = rec.get("valu_insts_steal_loop", 0.0)
     traffic = rec.get("hbm_bytes")
     t = kern_ms / 1e3
     achieved = valu / t / 1e9 if valu else None
     alg = st["alg_bytes"]
+    # frac counts every VALU instruction issued, the idle workers' steal
+    # loop included; frac_enumeration leaves that loop's share out
     return {"bound": "valu", "achieved": achieved, "peak": VALU_PEAK_GINST, "unit": "Ginst/s",
             "frac": achieved / VALU_PEAK_GINST if achieved else None, "traffic": traffic,
-            "valu_insts_per_launch": valu, "counters": "profiles/enum_kernel_motif4_chunglu18k.md",
+            "frac_enumeration": (valu - steal) / t / 1e9 / VALU_PEAK_GINST if valu else None,
+            "valu_insts_per_launch": valu, "valu_insts_steal_loop": steal,
+            "counters": "profiles/enum_kernel_motif4_chunglu18k.md",
             "hbm_GBs": traffic / t / 1e9 if traffic else None,
             "hbm_frac": traffic / t / 1e9 / HBM_PEAK_GBS if traffic else None,
             "alg_bytes": alg, "alg_GBs": alg / t / 1e9, "alg_frac_of_hbm_peak": alg / t / 1e9 / HBM_PEAK_GBS,
