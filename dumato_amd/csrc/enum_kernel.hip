@@ -1882,18 +1882,17 @@ __global__ void enum_init_kernel(SchedCounters *sc, unsigned long long njobs, in
     for (size_t t = i; t < nresults; t += stride) results[t] = 0;
     if (done_ticks)
         for (size_t t = i; t < (size_t)nq; t += stride) done_ticks[t] = 0;
-    // non-empty static queues (strided partition: queue w holds roots iff w < n)
+    // non-empty static queues (strided partition: queue w holds roots iff w
+    // < n), one queue per thread and one mask word per wave: a wave's lanes
+    // hold 64 consecutive queues (blocks of 256 threads, a stride of whole
+    // blocks), so the loop condition is uniform over the wave
     if (qmask) {
-        const size_t nwords = ((size_t)nqueues + 63) / 64;
-        for (size_t t = i; t < nwords; t += stride) {
-            unsigned long long m = 0;
-            for (int b = 0; b < 64; b++) {
-                const int64_t q = (int64_t)t * 64 + b;
-                if (q >= nqueues) break;
-                const bool ne = sq_off ? sq_off[q + 1] > sq_off[q] : q < strided_n;
-                m |= (unsigned long long)ne << b;
-            }
-            qmask[t] = m;
+        const size_t nbits = ((size_t)nqueues + 63) / 64 * 64;
+        for (size_t q = i; (q & ~(size_t)63) < nbits; q += stride) {
+            bool ne = false;
+            if (q < (size_t)nqueues) ne = sq_off ? sq_off[q + 1] > sq_off[q] : (int64_t)q < strided_n;
+            const unsigned long long m = __ballot(ne);
+            if ((q & 63) == 0) qmask[q >> 6] = m;
         }
     }
 }

@@ -8,6 +8,7 @@
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <climits>
 #include <cmath>
@@ -422,7 +423,9 @@ int run_enum(dm_graph *g, int K, bool motif, const dm_config &cfg, uint64_t *his
                 break;
             }
         std::copy(key, key + 8, np.key);
+        static std::atomic<uint64_t> part_serial{0};
         np.model_gen = g->model_gen;
+        np.serial = ++part_serial;
         np.skipped_bytes = skipped_bytes;
         if (learned) {
             if ((r = learned_partition(g, roots, K, motif, cfg, sh, np))) return r;
@@ -448,6 +451,7 @@ int run_enum(dm_graph *g, int K, bool motif, const dm_config &cfg, uint64_t *his
     // device workspaces
     size_t need_arena = (size_t)arena_bound(g, K, motif) * (size_t)sh.workers;
     if ((r = dev_grow(d->d_arena, d->arena_cap, need_arena))) return r;
+    const void *const old_bufs[4] = {d->d_sq_off, d->d_sq_jobs, d->d_pre_roots, d->d_pre_offs};
     if ((r = dev_grow(d->d_sq_off, d->sq_off_cap, sq_off.size()))) return r;
     if ((r = dev_grow(d->d_sq_jobs, d->sq_jobs_cap, std::max<size_t>(sq_jobs.size(), 4)))) return r;
     if ((r = dev_grow(d->d_sq_cur, d->sq_cur_cap, (size_t)sh.workers))) return r;
@@ -476,16 +480,36 @@ int run_enum(dm_graph *g, int K, bool motif, const dm_config &cfg, uint64_t *his
             return r;
     }
 
-    hipStream_t s = d->stream;
-    if (npre > 0) {
-        HIPCHK(hipMemcpyAsync(d->d_pre_roots, part->pre_roots.data(), (size_t)npre * 4, hipMemcpyHostToDevice, s));
-        HIPCHK(hipMemcpyAsync(d->d_pre_offs, part->pre_offs.data(), (size_t)npre * 8, hipMemcpyHostToDevice, s));
-        if ((r = fill_u32_launch(d->d_pre_cursor, 0, (size_t)npre, s))) return r;
+    // pinned staging for the counters and results read back after the launch
+    const size_t stage_need = sizeof(SchedCounters) + std::max<size_t>(nres, 1) * 8;
+    if (stage_need > d->stage_cap) {
+        if (d->h_stage) (void)hipHostFree(d->h_stage);
+        d->h_stage = nullptr;
+        d->stage_cap = 0;
+        HIPCHK(hipHostMalloc((void **)&d->h_stage, stage_need, hipHostMallocDefault));
+        d->stage_cap = stage_need;
     }
+
+    hipStream_t s = d->stream;
+    // the queues (and prebuilt-root list) go up once per partition: a
+    // repeated query finds them in place unless another partition, or a
+    // regrown buffer, has replaced them since
+    const void *const new_bufs[4] = {d->d_sq_off, d->d_sq_jobs, d->d_pre_roots, d->d_pre_offs};
+    if (!std::equal(old_bufs, old_bufs + 4, new_bufs)) d->up_serial = 0;
+    if (d->up_serial != part->serial) {
+        d->up_serial = 0;
+        if (npre > 0) {
+            HIPCHK(hipMemcpyAsync(d->d_pre_roots, part->pre_roots.data(), (size_t)npre * 4, hipMemcpyHostToDevice,
+                                  s));
+            HIPCHK(hipMemcpyAsync(d->d_pre_offs, part->pre_offs.data(), (size_t)npre * 8, hipMemcpyHostToDevice, s));
+        }
+        HIPCHK(hipMemcpyAsync(d->d_sq_off, sq_off.data(), sq_off.size() * 8, hipMemcpyHostToDevice, s));
+        if (!sq_jobs.empty())
+            HIPCHK(hipMemcpyAsync(d->d_sq_jobs, sq_jobs.data(), sq_jobs.size() * 4, hipMemcpyHostToDevice, s));
+        d->up_serial = part->serial;
+    }
+    if (npre > 0 && (r = fill_u32_launch(d->d_pre_cursor, 0, (size_t)npre, s))) return r;
     HIPCHK(hipMemcpyAsync(d->d_qd, &qd, sizeof(qd), hipMemcpyHostToDevice, s));
-    HIPCHK(hipMemcpyAsync(d->d_sq_off, sq_off.data(), sq_off.size() * 8, hipMemcpyHostToDevice, s));
-    if (!sq_jobs.empty())
-        HIPCHK(hipMemcpyAsync(d->d_sq_jobs, sq_jobs.data(), sq_jobs.size() * 4, hipMemcpyHostToDevice, s));
     if (want_cost && (r = fill_u32_launch(dg->d_root_cost, 0, (size_t)n * 2, s))) return r;
     if (want_cnt && (r = fill_u32_launch(dg->d_root_count, 0, (size_t)n * 2, s))) return r;
 
@@ -539,11 +563,13 @@ int run_enum(dm_graph *g, int K, bool motif, const dm_config &cfg, uint64_t *his
     if (npre > 0 && (r = prebuild_launch(p, s))) return r;
     if (njobs > 0 && (r = enum_launch(p, motif ? 1 : 0, sh.vwave, sh.blocks, sh.threads, sh.lds_total, s))) return r;
     HIPCHK(hipEventRecord(d->ev1, s));
-    SchedCounters sc1;
-    HIPCHK(hipMemcpyAsync(&sc1, d->d_sc, sizeof(sc1), hipMemcpyDeviceToHost, s));
-    std::vector<unsigned long long> h(nres);
-    HIPCHK(hipMemcpyAsync(h.data(), d->d_results, nres * 8, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(d->h_stage, d->d_sc, sizeof(SchedCounters), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(d->h_stage + sizeof(SchedCounters), d->d_results, nres * 8, hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
+    SchedCounters sc1;
+    memcpy(&sc1, d->h_stage, sizeof(sc1));
+    std::vector<unsigned long long> h(nres);
+    if (nres) memcpy(h.data(), d->h_stage + sizeof(SchedCounters), nres * 8);
     float kms = 0;
     HIPCHK(hipEventElapsedTime(&kms, d->ev0, d->ev1));
     if (sc1.error) return device_error(sc1.error, "enumeration kernel");

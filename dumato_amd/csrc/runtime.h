@@ -58,6 +58,12 @@ struct DeviceCtx {
     size_t pre_cursor_cap = 0;
     char *d_pre = nullptr;
     size_t pre_cap = 0;
+    // serial of the partition whose queues (and prebuilt-root list) the
+    // buffers above hold: a repeated query skips the upload
+    uint64_t up_serial = 0;
+    // pinned staging of a launch's counters and results (copied back per call)
+    char *h_stage = nullptr;
+    size_t stage_cap = 0;
 };
 
 // The current device's context (created, with its stream, events and queue,
@@ -112,6 +118,7 @@ struct dm_graph {
     struct Partition {
         int key[8];
         uint64_t model_gen;
+        uint64_t serial = 0;  // unique per computed partition (device upload cache)
         std::vector<int64_t> sq_off;
         std::vector<int32_t> sq_jobs;
         uint64_t skipped_bytes;
