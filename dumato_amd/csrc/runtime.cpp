@@ -401,16 +401,28 @@ int run_enum(dm_graph *g, int K, bool motif, const dm_config &cfg, uint64_t *his
         }
     }
     if (!part) {
-        std::vector<int32_t> roots;
-        roots.reserve(n);
-        uint64_t skipped_bytes = 0;  // motif root nodes with an empty extension set
-        for (int32_t v = 0; v < n; v++) {
-            int32_t orig = g->inv[v];
-            if (orig < rb || orig >= re) continue;
-            if (sample_mod > 1 && v % sample_mod != 0) continue;
-            if (root_has_work(g, v, K, motif)) roots.push_back(v);
-            else if (motif) skipped_bytes += 4ull * (uint64_t)(g->rcsr.off[v + 1] - g->rcsr.off[v]);
+        const int rkey[4] = {rb, re, sample_mod, motif ? 1 : 0};
+        std::shared_ptr<const dm_graph::RootList> rl;
+        for (const auto &c : g->root_lists)
+            if (std::equal(rkey, rkey + 4, c->key)) rl = c;
+        if (!rl) {
+            auto nl = std::make_shared<dm_graph::RootList>();
+            std::copy(rkey, rkey + 4, nl->key);
+            nl->roots.reserve(n);
+            nl->skipped_bytes = 0;  // motif root nodes with an empty extension set
+            for (int32_t v = 0; v < n; v++) {
+                int32_t orig = g->inv[v];
+                if (orig < rb || orig >= re) continue;
+                if (sample_mod > 1 && v % sample_mod != 0) continue;
+                if (root_has_work(g, v, K, motif)) nl->roots.push_back(v);
+                else if (motif) nl->skipped_bytes += 4ull * (uint64_t)(g->rcsr.off[v + 1] - g->rcsr.off[v]);
+            }
+            if (g->root_lists.size() >= 4) g->root_lists.pop_back();
+            g->root_lists.insert(g->root_lists.begin(), nl);
+            rl = nl;
         }
+        const std::vector<int32_t> &roots = rl->roots;
+        const uint64_t skipped_bytes = rl->skipped_bytes;
         dm_graph::Partition np;
         // a partition recomputed for the same key takes over the old one's
         // arrays: their capacity spares the new ones first-touch page faults

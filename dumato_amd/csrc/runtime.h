@@ -5,6 +5,7 @@
 
 #include <hip/hip_runtime_api.h>
 
+#include <memory>
 #include <mutex>
 #include <vector>
 
@@ -130,6 +131,17 @@ struct dm_graph {
         int64_t pre_bytes = 0;
     };
     std::vector<Partition> partitions;  // most recent first
+    // roots with work (and the node bytes of motif roots without) per
+    // (root range, sample step, motif): graph-derived, so a repartition
+    // reuses them
+    struct RootList {
+        int key[4];
+        std::vector<int32_t> roots;
+        uint64_t skipped_bytes;
+    };
+    // most recent first; a call holds its entry, so a nested call (the
+    // learned partition's feature count) may evict it safely
+    std::vector<std::shared_ptr<const RootList>> root_lists;
     uint64_t model_gen = 0;
     double last_infer_ms = 0, last_bin_ms = 0;  // phases of the last learned partition
 };
