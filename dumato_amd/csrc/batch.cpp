@@ -241,8 +241,8 @@ static int batch_build(dm_batch *b, const std::vector<dm_query> &qs, const dm_co
     for (int f = 0; f < 2; f++) {
         if (!has[f]) continue;
         Phase P;
-        P.mode = qs.size() == 1 ? 2 : f;
         P.kmax = kmax_f[f];
+        P.mode = qs.size() == 1 ? 2 : f == 1 ? launch_mode(true, P.kmax) : 0;
         P.wslots = f == 1 ? hslots : 0;
         for (size_t i = 0; i < qs.size(); i++)
             if ((qs[i].motif ? 1 : 0) == f) {
@@ -329,7 +329,7 @@ static int batch_build(dm_batch *b, const std::vector<dm_query> &qs, const dm_co
         p.sq_off = b->d_sq_off + o_off;
         p.sq_jobs = b->d_sq_jobs + o_jobs;
         p.sq_cur = b->d_sq_cur;
-        p.qmask = P.mode == 1 ? b->d_qmask : nullptr;  // motif phases only, as in single-graph calls
+        p.qmask = P.mode == 1 || P.mode == 3 ? b->d_qmask : nullptr;  // motif phases only, as in single-graph calls
         p.q = d->d_q;
         p.q_cap = (uint32_t)d->q_cap_alloc;
         p.epoch = d->d_epoch;

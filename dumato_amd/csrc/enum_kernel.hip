@@ -46,11 +46,15 @@ namespace dm {
 
 // Largest block (4 waves); a launch may use 1, 2 or 4 waves per block.
 constexpr int kThreads = 256;
-// Waves per SIMD each instantiation is compiled for: 4 (at most 128 VGPRs)
-// for the full-wave motif kernel, which fits that without spilling, 3 (168
-// VGPRs) for the others.
+// Waves per SIMD each instantiation is compiled for: 3 (168 VGPRs) for the
+// clique and mixed kernels, 4 (128 VGPRs) for the full-wave motif kernel.
+// Motif sizes k >= 5 get a second full-wave motif kernel at 5 waves (96
+// VGPRs, more of them spilled): their deeper searches wait on more loads, and
+// the extra wave hides them (Barabasi-Albert k = 5: 5.16 -> 4.74 s), while
+// k = 4 is faster at 4 (Chung-Lu 18K: 4.66 ms at 4, 4.96 ms at 5).
 template <int W, int MODE>
 constexpr int min_waves() { return W == 64 && MODE == 1 ? 4 : 3; }
+constexpr int kDeepMotifWaves = 5;
 constexpr int kMaxPm = 32;  // 2^(K-2) mask values for K <= 7
 
 // ---------------------------------------------------------------------------
@@ -1543,9 +1547,10 @@ DEV void flush_query(const EnumParams &p, const Grp<W> &g, int q, unsigned long 
     }
 }
 
-// MODE 0: cliques only, 1: motifs only, 2: mixed (batch launches).
-template <int W, int MODE>
-__global__ __launch_bounds__(kThreads, (min_waves<W, MODE>())) void enum_kernel(EnumParams p) {
+// MODE 0: cliques only, 1: motifs only, 2: mixed (batch launches); WAVES:
+// the waves per SIMD the registers are budgeted for.
+template <int W, int MODE, int WAVES = min_waves<W, MODE>()>
+__global__ __launch_bounds__(kThreads, WAVES) void enum_kernel(EnumParams p) {
     extern __shared__ __align__(16) char smem[];
     Grp<W> g;
     const int nthr = (int)blockDim.x;  // 64, 128 or 256
@@ -1907,12 +1912,17 @@ size_t enum_lds_bytes(int vwave, int threads, int kmax, int ncls_single, int his
     return b;
 }
 
+// launch modes: 0 cliques, 1 motifs, 2 mixed, 3 motifs of size >= 5 (the
+// full-wave motif kernel at kDeepMotifWaves; narrower widths keep mode 1's)
 template <int W>
 static const void *kernel_mode(int mode) {
     switch (mode) {
         case 0: return (const void *)enum_kernel<W, 0>;
         case 1: return (const void *)enum_kernel<W, 1>;
         case 2: return (const void *)enum_kernel<W, 2>;
+        case 3:
+            if constexpr (W == 64) return (const void *)enum_kernel<W, 1, kDeepMotifWaves>;
+            else return (const void *)enum_kernel<W, 1>;
         default: return nullptr;
     }
 }
@@ -1946,7 +1956,7 @@ int enum_occupancy(int vwave, int mode, int threads, size_t lds, int *blocks_per
     // leaves 2), so a persistent launch sized by the query alone would hold
     // blocks that only become resident once others have exited.
     constexpr size_t kLdsGranule = 1280;
-    static std::atomic<int> static_lds[4][3] = {};  // + 1: 0 = not read yet
+    static std::atomic<int> static_lds[4][4] = {};  // + 1: 0 = not read yet
     const int vi = vwave == 8 ? 0 : vwave == 16 ? 1 : vwave == 32 ? 2 : 3;
     int sl = static_lds[vi][mode].load(std::memory_order_relaxed) - 1;
     if (sl < 0) {

@@ -194,7 +194,9 @@ constexpr int job_steal_min(int32_t q) { return (q >> kQueryBits) & 0xff; }
 constexpr int32_t make_job_query(int q, int steal_min) { return q | (steal_min << kQueryBits); }
 
 // kernels (enum_kernel.hip)
-// mode: 0 cliques only, 1 motifs only, 2 mixed (batch)
+// mode: 0 cliques only, 1 motifs only, 2 mixed (batch), 3 motifs of size >= 5
+// (a motif kernel budgeted for one more wave per SIMD)
+inline int launch_mode(bool motif, int K) { return motif ? (K >= 5 ? 3 : 1) : 0; }
 size_t enum_lds_bytes(int vwave, int threads, int kmax, int ncls_single, int hist_slots, int lds_per_worker);
 int enum_occupancy(int vwave, int mode, int threads, size_t lds, int *blocks_per_cu);
 int enum_launch(const EnumParams &p, int mode, int vwave, int blocks, int threads, size_t lds,

@@ -188,7 +188,8 @@ static int noalloc_count(dm_graph *g, bool motif, int K, const dm_config &cfg, u
     const int queues = (int)std::min<int64_t>(n, sh.workers);
     if ((r = enum_init_launch(p, njobs, queues, (int32_t *)(sp + kSoEpoch), res, nres, nullptr, 0, s))) return r;
     if ((e = hipEventRecord(d->ev0, s)) != hipSuccess) return set_error(DM_ERR_HIP, "event: %s", hipGetErrorString(e));
-    if (njobs > 0 && (r = enum_launch(p, motif ? 1 : 0, sh.vwave, sh.blocks, sh.threads, sh.lds_total, s))) return r;
+    if (njobs > 0 && (r = enum_launch(p, launch_mode(motif, K), sh.vwave, sh.blocks, sh.threads, sh.lds_total, s)))
+        return r;
     if ((e = hipEventRecord(d->ev1, s)) != hipSuccess) return set_error(DM_ERR_HIP, "event: %s", hipGetErrorString(e));
     SchedCounters sc1;
     e = hipMemcpyAsync(&sc1, sp + kSoSc, sizeof(sc1), hipMemcpyDeviceToHost, s);

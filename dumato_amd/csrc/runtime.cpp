@@ -255,12 +255,12 @@ int pick_shape(dm_graph *g, int K, bool motif, const dm_config &cfg, Shape &sh) 
     int ncls = 0;
     if (motif) ncls = pattern_table(K)->ncls;
     // a tuned LDS size belongs to the full-size block it was tuned with
-    const int dl = max_lds_per_worker(vw, threads, motif ? 1 : 0, K, ncls, 0);
+    const int dl = max_lds_per_worker(vw, threads, launch_mode(motif, K), K, ncls, 0);
     int lds = cfg.lds_bytes > 0 ? cfg.lds_bytes : threads == 256 ? tuned_lds(g, K, motif, vw, dl) : dl;
     lds = (lds + 15) & ~15;
     size_t lds_total = enum_lds_bytes(vw, threads, K, ncls, 0, lds);
     int bpc = 0;
-    if ((r = enum_occupancy(vw, motif ? 1 : 0, threads, lds_total, &bpc))) return r;
+    if ((r = enum_occupancy(vw, launch_mode(motif, K), threads, lds_total, &bpc))) return r;
     if (bpc < 1) return set_error(DM_ERR_ARG, "launch does not fit one CU (LDS %zu B)", lds_total);
     sh.vwave = vw;
     sh.threads = threads;
@@ -573,7 +573,8 @@ int run_enum(dm_graph *g, int K, bool motif, const dm_config &cfg, uint64_t *his
     // the kernel time includes the bitmap pre-pass of a learned partition
     HIPCHK(hipEventRecord(d->ev0, s));
     if (npre > 0 && (r = prebuild_launch(p, s))) return r;
-    if (njobs > 0 && (r = enum_launch(p, motif ? 1 : 0, sh.vwave, sh.blocks, sh.threads, sh.lds_total, s))) return r;
+    if (njobs > 0 && (r = enum_launch(p, launch_mode(motif, K), sh.vwave, sh.blocks, sh.threads, sh.lds_total, s)))
+        return r;
     HIPCHK(hipEventRecord(d->ev1, s));
     HIPCHK(hipMemcpyAsync(d->h_stage, d->d_sc, sizeof(SchedCounters), hipMemcpyDeviceToHost, s));
     HIPCHK(hipMemcpyAsync(d->h_stage + sizeof(SchedCounters), d->d_results, nres * 8, hipMemcpyDeviceToHost, s));

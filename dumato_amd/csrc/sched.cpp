@@ -100,7 +100,7 @@ int autotune(dm_graph *g, int K, bool motif, double budget_ms, dm_config *best_o
         }
     }
     const int ncls = motif ? pattern_table(K)->ncls : 0;
-    const int dl = max_lds_per_worker(bc.vwave, 256, motif ? 1 : 0, K, ncls, 0);
+    const int dl = max_lds_per_worker(bc.vwave, 256, launch_mode(motif, K), K, ncls, 0);
     if (full_search) {
         const int d8 = 32768 / (256 / bc.vwave);  // 8 KiB for a full wave
         for (int lds : {(dl / 2) & ~255, std::min(d8, dl), dl}) {
