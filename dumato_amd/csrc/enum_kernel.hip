@@ -48,13 +48,15 @@ namespace dm {
 constexpr int kThreads = 256;
 // Waves per SIMD each instantiation is compiled for: 3 (168 VGPRs) for the
 // clique and mixed kernels, 4 (128 VGPRs) for the full-wave motif kernel.
-// Motif sizes k >= 5 get a second full-wave motif kernel at 5 waves (96
+// Motif sizes k >= 5 get a second full-wave motif kernel at 6 waves (80
 // VGPRs, more of them spilled): their deeper searches wait on more loads, and
-// the extra wave hides them (Barabasi-Albert k = 5: 5.16 -> 4.74 s), while
-// k = 4 is faster at 4 (Chung-Lu 18K: 4.66 ms at 4, 4.96 ms at 5).
+// the extra waves hide them (Barabasi-Albert k = 5: 5.16 s at 4 waves, 4.73 s
+// at 5, 4.53 s at 6, 4.54 s at 7, 5.52 s at 8), while k = 4 is faster at 4
+// (Chung-Lu 18K: 4.66 ms at 4, 4.96 ms at 5).  The clique kernel is slower
+// at 4 than at 3 (R-MAT k = 5: 1.42 -> 1.66 ms).
 template <int W, int MODE>
 constexpr int min_waves() { return W == 64 && MODE == 1 ? 4 : 3; }
-constexpr int kDeepMotifWaves = 5;
+constexpr int kDeepMotifWaves = 6;
 constexpr int kMaxPm = 32;  // 2^(K-2) mask values for K <= 7
 
 // ---------------------------------------------------------------------------
