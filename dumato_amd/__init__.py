@@ -47,7 +47,8 @@ class Stats(ctypes.Structure):
                 ("blocks", ctypes.c_int32), ("lds_bytes", ctypes.c_int32),
                 ("first_job_ms", ctypes.c_double), ("static_done_ms", ctypes.c_double),
                 ("last_job_ms", ctypes.c_double), ("exit_ms", ctypes.c_double),
-                ("job_max_cycles", ctypes.c_uint64), ("infer_ms", ctypes.c_double), ("bin_ms", ctypes.c_double)]
+                ("job_max_cycles", ctypes.c_uint64), ("infer_ms", ctypes.c_double), ("bin_ms", ctypes.c_double),
+                ("active_workers", ctypes.c_uint64)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}

@@ -421,6 +421,7 @@ static int batch_run(dm_batch *b, const std::vector<dm_query> &qs, const dm_conf
         sc1.busy_sum += c.busy_sum;
         sc1.busy_max = std::max(sc1.busy_max, c.busy_max);
         sc1.job_max = std::max(sc1.job_max, c.job_max);
+        sc1.active_workers = std::max(sc1.active_workers, c.active_workers);
         sc1.t_first_job = c.t_first_job;
         sc1.t_static_done = c.t_static_done;
         sc1.t_last_job = c.t_last_job;
@@ -460,6 +461,7 @@ static int batch_run(dm_batch *b, const std::vector<dm_query> &qs, const dm_conf
         st.exit_ms = rel(sc1.t_exit);
         st.job_max_cycles = sc1.job_max;
     }
+    st.active_workers = sc1.active_workers;
     return DM_OK;
 }
 

@@ -1704,9 +1704,10 @@ __global__ __launch_bounds__(kThreads, WAVES) void enum_kernel(EnumParams p) {
     // Per-worker statistics (held by lane 0 of each worker) reduced over the
     // block, so that each block, not each worker, adds to the shared words.
     // The reduction reuses the start of the block's dynamic LDS (at least the
-    // worker records, > 80 B per wave): a static array would count towards
-    // the LDS that decides how many blocks share a CU.
-    constexpr int NF = 10;
+    // worker records, > 88 B per wave): a static array would count towards
+    // the LDS that decides how many blocks share a CU.  Fields 0-4 and 10 are
+    // sums, the others maxima.
+    constexpr int NF = 11;
     unsigned long long (*red)[NF] = (unsigned long long (*)[NF])smem;
     const bool holder = wid < p.nworkers && g.gl == 0;
     const unsigned long long *st = ws.st();
@@ -1715,13 +1716,14 @@ __global__ __launch_bounds__(kThreads, WAVES) void enum_kernel(EnumParams p) {
                                 holder && single ? st[kStQCount] : 0ull,
                                 holder ? st[kStBusy] : 0ull,    holder ? st[kStFirst] : 0ull,
                                 holder ? st[kStStatic] : 0ull,  holder ? st[kStLast] : 0ull,
-                                holder ? st[kStJobMax] : 0ull};
+                                holder ? st[kStJobMax] : 0ull,
+                                holder && st[kStJobs] ? 1ull : 0ull};
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
 #pragma unroll
         for (int i = 0; i < NF; i++) {
             const unsigned long long y = __shfl_xor(f[i], o, 64);
-            f[i] = i < 5 ? f[i] + y : (y > f[i] ? y : f[i]);
+            f[i] = i < 5 || i == 10 ? f[i] + y : (y > f[i] ? y : f[i]);
         }
     }
     // every wave has read its records (and the class counters were flushed)
@@ -1734,7 +1736,8 @@ __global__ __launch_bounds__(kThreads, WAVES) void enum_kernel(EnumParams p) {
     __syncthreads();
     if (threadIdx.x == 0) {
         for (int w = 1; w < nthr / 64; w++) {
-            for (int i = 0; i < NF; i++) f[i] = i < 5 ? f[i] + red[w][i] : (red[w][i] > f[i] ? red[w][i] : f[i]);
+            for (int i = 0; i < NF; i++)
+                f[i] = i < 5 || i == 10 ? f[i] + red[w][i] : (red[w][i] > f[i] ? red[w][i] : f[i]);
         }
         if (f[0]) atomicAdd(&p.sc->alg_bytes, f[0]);
         if (f[1]) atomicAdd(&p.sc->jobs, f[1]);
@@ -1746,6 +1749,7 @@ __global__ __launch_bounds__(kThreads, WAVES) void enum_kernel(EnumParams p) {
         atomicMax(&p.sc->t_static_done, f[7]);
         atomicMax(&p.sc->t_last_job, f[8]);
         atomicMax(&p.sc->job_max, f[9]);
+        if (f[10]) atomicAdd(&p.sc->active_workers, f[10]);
         atomicMax(&p.sc->t_exit, (unsigned long long)wall_clock64());
     }
 }
@@ -1882,6 +1886,7 @@ __global__ void enum_init_kernel(SchedCounters *sc, unsigned long long njobs, in
         sc->t_last_job = 0;
         sc->t_exit = 0;
         sc->job_max = 0;
+        sc->active_workers = 0;
         *epoch = *epoch + 1;
         if (start_ticks) *start_ticks = ~0ull;
     }

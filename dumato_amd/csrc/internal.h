@@ -103,6 +103,7 @@ struct SchedCounters {
     // job and to exit
     unsigned long long t_start, t_first_job, t_static_done, t_last_job, t_exit;
     unsigned long long job_max;  // cycles of the longest single job
+    unsigned long long active_workers;  // workers that ran at least one job
 };
 
 // One counting query of a launch.  A single-graph call has one; a batch

@@ -212,6 +212,7 @@ static int noalloc_count(dm_graph *g, bool motif, int K, const dm_config &cfg, u
     st.busy_sum_cycles = sc1.busy_sum;
     st.busy_max_cycles = sc1.busy_max;
     st.job_max_cycles = sc1.job_max;
+    st.active_workers = sc1.active_workers;
     st.vwave = sh.vwave;
     st.workers = sh.workers;
     st.blocks = sh.blocks;

@@ -637,6 +637,7 @@ int run_enum(dm_graph *g, int K, bool motif, const dm_config &cfg, uint64_t *his
         st.exit_ms = rel(sc1.t_exit);
         st.job_max_cycles = sc1.job_max;
     }
+    st.active_workers = sc1.active_workers;
     st.vwave = sh.vwave;
     st.workers = sh.workers;
     st.blocks = sh.blocks;

@@ -80,6 +80,8 @@ typedef struct dm_stats {
                                  on the makespan that stealing cannot cut) */
     double infer_ms;          /* learned: feature scaling and the MLP forward launch */
     double bin_ms;            /* learned: splitting, ordering and binning the roots */
+    uint64_t active_workers;  /* workers that ran at least one job (a batch:
+                                 the larger of its two launches) */
 } dm_stats;
 
 /* Fills *cfg with the defaults (plain scheduler, autotuned width/LDS). */

@@ -50,6 +50,33 @@ def test_timeline_is_ordered(rmat):
     assert st["jobs"] >= 1 and st["busy_max_cycles"] > 0
 
 
+def test_every_launched_worker_runs(rmat, golden):
+    """A default launch holds only as many blocks as fit on the CUs together.
+    gfx950 allocates a block's LDS in 1,280-byte units; a grid sized by the
+    LDS request alone holds blocks that become resident only when others exit,
+    and whose workers then find nothing left to run.  With more roots than
+    workers, every worker of the launch runs jobs: cliques (3 blocks per CU),
+    motifs k = 4 (4 per CU) and k >= 5 (the motif kernel at 6 waves)."""
+    for k in (4, 5, 6):
+        print("case all workers run: rmat clique k=%d" % k, flush=True)
+        assert rmat.clique_count(k, cfg()) == golden["rmat_s17"]["clique"][str(k)]
+        st = rmat.last_stats()
+        assert st["active_workers"] == st["workers"], (k, st["active_workers"], st["workers"])
+    g = dm.Graph.erdos_renyi(20000, 60000, 777)
+    off, adj = g.csr()
+    src = np.repeat(np.arange(len(off) - 1), np.diff(off)).astype(np.int32)
+    keep = src < adj
+    ref = O.csr_from_edges(src[keep], adj[keep], len(off) - 1)
+    for k in (4, 5):
+        print("case all workers run: ER 20000/60000 motif k=%d" % k, flush=True)
+        tot, hist = g.motif_count(k, cfg())
+        st = g.last_stats()
+        wt, wh = O.motif_count(ref, k)
+        assert tot == wt and np.array_equal(hist, wh)
+        assert st["workers"] >= (6144 if k >= 5 else 4096)
+        assert st["active_workers"] == st["workers"], (k, st["active_workers"], st["workers"])
+
+
 @pytest.mark.parametrize("period", [0, 2, 50])
 def test_donation_check_period_keeps_counts(citeseer, period):
     g, ref = citeseer
