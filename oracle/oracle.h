@@ -2,8 +2,9 @@
  * CPU oracle for DuMato's enumeration path (clique and motif counting).
  *
  * TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and the
- * cpu_baseline leg of bench.py may load this library, and only as the checker
- * (or, for bench.py, as the timed CPU port of the reference's DFS logic).
+ * cpu_baseline leg of bench.py may load this library, and only to check the
+ * GPU results against (or, for bench.py, as the timed CPU port of the
+ * reference's DFS logic).
  * The product library (dumato_amd/) never links or calls it.
  *
  * Every function restates a reference function; each definition in oracle.c

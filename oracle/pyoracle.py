@@ -1,8 +1,8 @@
 """ctypes wrapper of the CPU oracle (oracle/liboracle.so).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
-cpu_baseline leg of bench.py, as the checker.  The product package never
-imports it.
+cpu_baseline leg of bench.py, to check the GPU results against.  The product
+package never imports it.
 
 The pattern tables come from the committed fixtures tests/golden/dict_k.json,
 which hold the reference's own dictionaries (dictionaries/k.csv).
