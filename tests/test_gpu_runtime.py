@@ -144,3 +144,37 @@ def test_partition_cache_follows_root_range_and_model(citeseer):
     g.sched_train(False, 3, epochs=5, seed=2)
     b = g.clique_count(4, cfg(sched=dm.DM_SCHED_LEARNED))
     assert a == b == full
+
+
+def test_cached_partitions_alternate_across_graphs(citeseer):
+    """A repeated query reuses its partition and, when no other partition has
+    replaced them since, the queues already on the device.  Alternating graphs,
+    families, sizes, root ranges and schedulers on one device must upload
+    whatever the next launch needs: every count stays exact."""
+    g1, ref1 = citeseer
+    g2 = dm.Graph.erdos_renyi(3000, 15000, 4242)
+    off, adj = g2.csr()
+    src = np.repeat(np.arange(len(off) - 1), np.diff(off)).astype(np.int32)
+    keep = src < adj
+    ref2 = O.csr_from_edges(src[keep], adj[keep], len(off) - 1)
+    want = {}
+    for name, ref in (("g1", ref1), ("g2", ref2)):
+        for k in (3, 4):
+            want[(name, "c", k)] = O.clique_count(ref, k)
+            want[(name, "m", k)] = O.motif_count(ref, k)
+    n1 = g1.info()[0]
+    part_cfg = cfg(root_begin=n1 // 3, root_end=n1)
+    part_want = g1.clique_count(4, part_cfg)
+    g2.sched_train(False, 3, epochs=5, seed=3)
+    for it in range(3):
+        for name, g in (("g1", g1), ("g2", g2)):
+            for k in (3, 4):
+                print("case alternate it=%d %s k=%d" % (it, name, k), flush=True)
+                assert g.clique_count(k, cfg()) == want[(name, "c", k)]
+                tot, hist = g.motif_count(k, cfg())
+                wt, wh = want[(name, "m", k)]
+                assert tot == wt and np.array_equal(hist, wh)
+        assert g1.clique_count(4, part_cfg) == part_want
+        assert g2.clique_count(4, cfg(sched=dm.DM_SCHED_LEARNED)) == want[("g2", "c", 4)]
+        # a recomputed partition of the same key replaces the one on the device
+        assert g2.clique_count(4, cfg(repartition=1)) == want[("g2", "c", 4)]
