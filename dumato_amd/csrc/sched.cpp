@@ -416,6 +416,33 @@ static int predict_all(dm_graph *g, const float **out) {
 // from in order (dynamic longest-first list scheduling).  The orderings are
 // counting sorts (exact only over the heavy head), so the host cost stays
 // linear in the number of roots.
+// Ascending sort of non-negative vertex ids: two radix passes of 11 bits for
+// ids below 2^22, std::sort otherwise.
+static void sort_ids(std::vector<int32_t> &a) {
+    int32_t mx = 0;
+    for (int32_t x : a) mx = std::max(mx, x);
+    if (mx >= (1 << 22) || a.size() < 64) {
+        std::sort(a.begin(), a.end());
+        return;
+    }
+    static thread_local std::vector<int32_t> tmp;
+    tmp.resize(a.size());
+    int32_t cnt[2048];
+    for (int pass = 0; pass < 2; pass++) {
+        const int sh = 11 * pass;
+        std::vector<int32_t> &src = pass == 0 ? a : tmp, &dst = pass == 0 ? tmp : a;
+        std::fill(cnt, cnt + 2048, 0);
+        for (int32_t x : src) cnt[(x >> sh) & 2047]++;
+        int32_t acc = 0;
+        for (int b = 0; b < 2048; b++) {
+            const int32_t c = cnt[b];
+            cnt[b] = acc;
+            acc += c;
+        }
+        for (int32_t x : src) dst[cnt[(x >> sh) & 2047]++] = x;
+    }
+}
+
 int learned_partition(dm_graph *g, const std::vector<int32_t> &roots, int K, bool motif, const dm_config &cfg,
                       const Shape &sh, dm_graph::Partition &part) {
     if (!g->model) return set_error(DM_ERR_ARG, "learned scheduler: train a model first (dm_sched_train)");
@@ -475,13 +502,28 @@ int learned_partition(dm_graph *g, const std::vector<int32_t> &roots, int K, boo
         g->growth_from = kt;
         g->growth_to = K;
     }
-    float pmin = 1e30f, pmax = -1e30f;
+    // predicted log cycles at the training size (carried up by the growth),
+    // then their range with 8 independent minima and maxima: one running
+    // pair inside the gather loop made it a serial dependency chain
+    const float *gr = grow ? g->growth.data() : nullptr;
     for (size_t i = 0; i < nr; i++) {
         const int32_t v = roots[i];
-        const float lc = z[v] * ys + ym;  // predicted log cycles at the training size
-        kv[i] = grow ? lc + g->growth[v] : lc;
-        pmin = std::min(pmin, kv[i]);
-        pmax = std::max(pmax, kv[i]);
+        const float lc = z[v] * ys + ym;
+        kv[i] = gr ? lc + gr[v] : lc;
+    }
+    float pmin = 1e30f, pmax = -1e30f;
+    {
+        float mn[8], mx[8];
+        for (int j = 0; j < 8; j++) mn[j] = 1e30f, mx[j] = -1e30f;
+        size_t i = 0;
+        for (; i + 8 <= nr; i += 8)
+            for (int j = 0; j < 8; j++) {
+                const float x = kv[i + j];
+                mn[j] = x < mn[j] ? x : mn[j];
+                mx[j] = x > mx[j] ? x : mx[j];
+            }
+        for (; i < nr; i++) pmin = std::min(pmin, kv[i]), pmax = std::max(pmax, kv[i]);
+        for (int j = 0; j < 8; j++) pmin = std::min(pmin, mn[j]), pmax = std::max(pmax, mx[j]);
     }
     constexpr int NB = 4096;
     const float scale = pmax > pmin ? (float)(NB - 1) / (pmax - pmin) : 0.f;
@@ -627,7 +669,7 @@ int learned_partition(dm_graph *g, const std::vector<int32_t> &roots, int K, boo
         put(roots[idx[t]], 0, INT_MAX, halves && t >= nr / 2 ? make_job_query(0, 16) : 0);
 
     // prebuilt bitmaps, ascending by root
-    std::sort(part.pre_roots.begin(), part.pre_roots.end());
+    sort_ids(part.pre_roots);
     part.pre_offs.resize(part.pre_roots.size());
     int64_t off = 0;
     for (size_t i = 0; i < part.pre_roots.size(); i++) {
