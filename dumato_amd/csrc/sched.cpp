@@ -7,6 +7,7 @@
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <climits>
 #include <cmath>
@@ -540,10 +541,14 @@ int learned_partition(dm_graph *g, const std::vector<int32_t> &roots, int K, boo
         for (size_t i = 0; i < nr; i++) idx[fill[bk[i]]++] = (int32_t)i;
     }
     if (H > 0) {
-        const size_t hend = (size_t)bstart[bk[idx[H - 1]] + 1];
-        std::sort(idx.begin(), idx.begin() + hend, [&](int32_t a, int32_t b) {
-            return kv[a] > kv[b] || (kv[a] == kv[b] && a < b);
-        });
+        // buckets are monotone in the cost (equal costs share a bucket), so
+        // sorting each bucket of the head on its own orders the head exactly
+        const int hb_last = bk[idx[H - 1]];
+        for (int b = 0; b <= hb_last; b++)
+            if (bstart[b + 1] - bstart[b] > 1)
+                std::sort(idx.begin() + bstart[b], idx.begin() + bstart[b + 1], [&](int32_t a, int32_t c) {
+                    return kv[a] > kv[c] || (kv[a] == kv[c] && a < c);
+                });
     }
 
     const bool can_split = !(K == 2 && !motif);
@@ -567,6 +572,12 @@ int learned_partition(dm_graph *g, const std::vector<int32_t> &roots, int K, boo
     // root's less log P (the cuts give the P pieces equal estimated work)
     static thread_local std::vector<Job> jobs;
     jobs.clear();
+    // log of a root's piece count (at most 64)
+    static const auto logp = [] {
+        std::array<float, 65> t{};
+        for (int i = 1; i <= 64; i++) t[i] = (float)std::log((double)i);
+        return t;
+    }();
     part.pre_roots.clear();
     for (size_t rk = 0; rk < H; rk++) {
         const int32_t v = roots[idx[rk]];
@@ -578,7 +589,7 @@ int learned_partition(dm_graph *g, const std::vector<int32_t> &roots, int K, boo
             jobs.push_back({v, 0, INT_MAX, q, kv[idx[rk]]});
             continue;
         }
-        const float lc = kv[idx[rk]] - (float)std::log((double)P);
+        const float lc = kv[idx[rk]] - logp[P];
         if (bitmap_path && nch <= 256) {
             // a prebuilt bitmap root: its jobs claim children from one shared
             // cursor, so no weight model fixes where its work is.  Below k = 6
@@ -587,7 +598,7 @@ int learned_partition(dm_graph *g, const std::vector<int32_t> &roots, int K, boo
             // 1.83 ms with P, 1.46 ms with P / 4)
             const int PS = K >= 6 ? P : std::max(2, P / 4);
             part.pre_roots.push_back(v);
-            const float lcs = kv[idx[rk]] - (float)std::log((double)PS);
+            const float lcs = kv[idx[rk]] - logp[PS];
             for (int pi = 0; pi < PS; pi++) jobs.push_back({v, -2, PS, q, lcs});
             continue;
         }
