@@ -204,7 +204,7 @@ int block_threads(const dm_config &cfg, int *threads) {
 
 static int max_lds_uncached(int vwave, int threads, int mode, int K, int ncls_single, int hist_slots);
 
-// Largest per-worker LDS stack (a multiple of 256 B, at most 16 KiB) that
+// Largest per-worker LDS stack (a multiple of 16 B, at most 16 KiB) that
 // keeps the occupancy the kernel's registers allow: the LDS of a CU (160
 // KiB) divided over that many blocks, less each block's fixed records.
 int max_lds_per_worker(int vwave, int threads, int mode, int K, int ncls_single, int hist_slots) {
@@ -339,7 +339,11 @@ void plain_partition(const dm_graph *g, const std::vector<int32_t> &roots, int K
     // descending key by a counting sort (equal keys keep their ascending
     // rank), then job t to worker t mod workers
     const size_t nr = roots.size(), W = (size_t)std::max(workers, 1);
-    std::vector<int32_t> key(nr);
+    // scratch keeps its capacity between calls (a fresh allocation's
+    // first-touch page faults cost as much as the sort)
+    static thread_local std::vector<int32_t> key, order;
+    static thread_local std::vector<int64_t> start;
+    key.resize(nr);
     int32_t kmax = 0;
     const int64_t *off = g->rcsr.off.data();
     const int32_t *adj = g->rcsr.adj.data();
@@ -354,20 +358,26 @@ void plain_partition(const dm_graph *g, const std::vector<int32_t> &roots, int K
         }
         kmax = std::max(kmax, key[i]);
     }
-    std::vector<int64_t> start((size_t)kmax + 2, 0);
+    start.assign((size_t)kmax + 2, 0);
     for (size_t i = 0; i < nr; i++) start[(size_t)(kmax - key[i]) + 1]++;
     for (size_t b = 1; b < start.size(); b++) start[b] += start[b - 1];
-    std::vector<int32_t> order(nr);
+    order.resize(nr);
     for (size_t i = 0; i < nr; i++) order[start[(size_t)(kmax - key[i])]++] = (int32_t)i;
     sq_off.assign(W + 1, 0);
     for (size_t w = 0; w < W; w++) sq_off[w + 1] = sq_off[w] + (int64_t)(nr > w ? (nr - 1 - w) / W + 1 : 0);
-    sq_jobs.assign(nr * 4, 0);
+    sq_jobs.resize(nr * 4);
+    // job t goes to worker t % W, slot t / W: kept as counters, no division
+    size_t w = 0, slot = 0;
     for (size_t t = 0; t < nr; t++) {
-        int32_t *jp = &sq_jobs[4 * (size_t)(sq_off[t % W] + (int64_t)(t / W))];
+        int32_t *jp = &sq_jobs[4 * (size_t)(sq_off[w] + (int64_t)slot)];
         jp[0] = roots[order[t]];
         jp[1] = 0;
         jp[2] = INT_MAX;
         jp[3] = 0;
+        if (++w == W) {
+            w = 0;
+            slot++;
+        }
     }
 }
 
