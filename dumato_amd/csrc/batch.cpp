@@ -151,8 +151,10 @@ static void batch_partition(const dm_batch *b, const std::vector<dm_query> &qs, 
     // alone, every graph's light pieces come last and every graph finishes
     // within 5% of the end; this way half of them finish in the first fifth
     // of the launch, and the heavy pieces start only that small share later
-    // (256-graph batch: p50 9.39 -> 1.72 s, makespan unchanged).
-    constexpr double kEarlyShare = 0.05;
+    // (256-graph batch: p50 9.39 -> 1.72 s, makespan unchanged, with a 5%
+    // share at 4 waves per SIMD).  With the 6-wave motif kernel a 5% share
+    // gave p50 2.42 s, 10-35% 2.04-2.05 s (makespan 8.98 -> 8.94 s), 2% 8.2 s.
+    constexpr double kEarlyShare = 0.20;
     {
         std::vector<double> gcost(b->ngraphs, 0.0);
         double total = 0;
